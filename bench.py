@@ -1,0 +1,199 @@
+"""Throughput of the fused shallow-water step on one MI355X.
+
+Workload: the 4096 x 4096 FP32 urban dam-break case (dx = 1 m, wet start
+h = 0.05 m with a 0.5 m square in the centre, no rain, cfl 0.45, one open edge),
+advanced by the device-resident adaptive loop: each step is one fused kernel
+(ghosts, reconstruction, HLL, rain, clip, friction, wet/dry mask) whose
+epilogue reduces the CFL maximum and FP64 mass partials and whose last
+workgroup sets the next dt.  Inputs are resident in HBM before timing starts.
+
+Prints ONE JSON line.  ``value`` = cell-updates/s over exactly ``--steps``
+timed steps (HIP events on the stream the kernels run on, bracketed by
+device synchronisation); ``roofline`` prices the step at 37 algorithmic bytes
+per cell-update against 8 TB/s; ``cpu_baseline`` times the NumPy FP32 oracle
+on a few steps of the same workload on the host (one thread).
+
+    python bench.py [--steps 2000] [--warmup 20] [--variant fast|reference]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-hybrid-flood-forecasting_amd"))
+
+METRIC = "cell-updates/s & %HBM roofline at 4096²; ensemble & adjoint vs plain step"
+BYTES_PER_CELL = 37  # h,hu,hv,bed,n,mult read (24) + mask (1) + h,hu,hv written (12)
+HBM_PEAK_GBS = 8000.0
+
+
+def _even_divisor(k: int, cap: int = 64) -> int:
+    for c in range(min(cap, k), 1, -1):
+        if k % c == 0 and c % 2 == 0:
+            return c
+    return 2
+
+
+def time_loop(params, h0, steps, warmup, variant, rows=None):
+    """Advance the device loop: ``warmup`` untimed steps, then ``steps`` timed."""
+    import ctypes
+
+    import torch
+
+    from hybrid_flood_amd import _lib
+    from hybrid_flood_amd import numerics as nm
+
+    dev = params.bed.device
+    lib = _lib.load()
+    ctx = nm.context_for(params, variant)
+    if rows:
+        ctx.set_tile_rows(rows)
+    chunk = _even_divisor(steps)
+    ctx.set_graph_chunk(chunk)
+    warm = max(chunk, int(math.ceil(warmup / chunk)) * chunk) if warmup > 0 else 0
+    h = torch.as_tensor(h0, device=dev).contiguous()
+    state = nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h))
+    s = _lib.stream_handle(dev)
+    st = _lib.state_struct(*state)
+    _lib.check(lib.swe_loop_begin(ctx.handle, st, 0.0, warm + steps, s), "swe_loop_begin")
+    if warm:
+        _lib.check(lib.swe_loop_advance(ctx.handle, warm, s), "swe_loop_advance")
+    torch.cuda.synchronize(dev)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    _lib.check(lib.swe_loop_advance(ctx.handle, steps, s), "swe_loop_advance")
+    ev1.record()
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1)
+    info = _lib.SweRunInfo()
+    _lib.check(lib.swe_loop_end(ctx.handle, st, ctypes.byref(info), s), "swe_loop_end")
+    if info.steps_used != warm + steps:
+        raise RuntimeError(f"device loop took {info.steps_used} steps, expected {warm + steps}")
+    return {"ms": ms, "wall_s": wall, "warmup": warm, "info": info.as_dict(), "chunk": chunk}
+
+
+def cpu_baseline(case, steps):
+    """NumPy FP32 oracle on ``steps`` adaptive steps of the same workload (one thread)."""
+    import numpy as np
+
+    sys.path.insert(0, ROOT)
+    from oracle import swe_oracle as o
+
+    p = o.OracleParams(bed=case.bed, manning_n=case.manning_n, domain_mask=case.domain_mask,
+                       boundary_types=case.boundary_types, rainfall_times_s=case.rain_times_s,
+                       rainfall_rates_m_s=case.rain_rates_m_s, dx=case.dx, cfl=case.cfl,
+                       dry_tolerance=case.dry_tolerance, max_dt=case.max_dt)
+    z = np.zeros_like(case.h0)
+    t0 = time.perf_counter()
+    o.run_steps(o.OracleState(case.h0, z, z), p, steps)
+    el = time.perf_counter() - t0
+    n = case.bed.size
+    return {
+        "value": n * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+        "sample": f"{steps} adaptive steps (CFL dt + step) of the same {case.bed.shape[0]}x"
+                  f"{case.bed.shape[1]} workload, NumPy FP32 oracle, single thread, {el:.1f} s",
+        "host_cpu_count": os.cpu_count(),
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--variant", default="fast", choices=["fast", "reference"])
+    ap.add_argument("--rows", type=int, default=0, help="rows per wavefront strip (0 = default)")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for cpu_baseline (0 = skip)")
+    ap.add_argument("--extra", action="store_true", help="also time the other kernel variant")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
+              "(the step is measured on one MI355X; multi-GPU slab sharding is design only).",
+              file=sys.stderr)
+        return 2
+    if args.steps < 2:
+        print("bench.py: --steps must be >= 2", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd import synthetic as sy
+
+    dev = torch.device("cuda", 0)
+    case = sy.dam_break_urban_case(args.size, n_steps=args.steps)
+    params = nm.make_params(case.bed, case.manning_n, case.domain_mask,
+                            boundary_types=case.boundary_types,
+                            rainfall_times_s=case.rain_times_s,
+                            rainfall_rates_m_s=case.rain_rates_m_s, dx=case.dx, cfl=case.cfl,
+                            dry_tolerance=case.dry_tolerance, max_dt=case.max_dt, device=dev)
+    cells = case.bed.size
+    r = time_loop(params, case.h0, args.steps, args.warmup, args.variant, args.rows)
+    ms_per_step = r["ms"] / args.steps
+    value = cells * args.steps / (r["ms"] * 1e-3)
+    achieved = BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": r["warmup"],
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"dam_break_urban_{args.size}: {args.size}x{args.size} FP32 cells, dx=1 m, "
+                        "wet start 0.05 m + 0.5 m central square, no rain, cfl 0.45, "
+                        "adaptive dt, fused step + CFL/mass epilogue",
+            "cells": cells,
+            "kernel_variant": args.variant,
+            "graph_chunk": r["chunk"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_cell_update": BYTES_PER_CELL,
+        },
+        "wall_s": r["wall_s"],
+        "mass_balance_rel": r["info"]["mass_balance_rel"],
+        "simulated_s": r["info"]["final_time_s"],
+    }
+    prof = os.path.join(ROOT, "profiles", "step_fast_4096_traffic.json")
+    if args.variant == "fast" and os.path.exists(prof):
+        with open(prof) as f:
+            t = json.load(f)
+        line["roofline"]["traffic"] = t.get("hbm_bytes_per_launch")
+        line["roofline"]["traffic_source"] = "profiles/step_fast_4096_traffic.json (rocprofv3 --pmc)"
+    if args.extra:
+        other = "reference" if args.variant == "fast" else "fast"
+        k = min(args.steps, 400)
+        ro = time_loop(params, case.h0, k, args.warmup, other, args.rows)
+        line["other_variant"] = {"variant": other, "steps": k,
+                                 "cell_updates_per_s": cells * k / (ro["ms"] * 1e-3),
+                                 "ms_per_step": ro["ms"] / k}
+    if args.cpu_steps > 0:
+        line["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

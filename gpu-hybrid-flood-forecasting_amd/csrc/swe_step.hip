@@ -1,0 +1,1036 @@
+// libswe_hip: fused shallow-water step, CFL reduction and device time loop
+// for MI355X (gfx950), behind the C ABI declared in include/swe_hip.h.
+//
+// Kernels
+//   k_step<M, MODE>    one fused pass per step: ghost cells, wall mirroring,
+//                      hydrostatic reconstruction, HLL on every face once,
+//                      divergence, rain, positivity clip, semi-implicit
+//                      Manning friction and wet/dry masking
+//                      (jax_solver/numerics.py:312-369).  In MODE_LOOP the
+//                      epilogue also reduces the CFL maximum of the new state
+//                      and the FP64 volume, clip and open-boundary partials;
+//                      the last workgroup to finish sets the next dt, time,
+//                      output slots and stop flag (numerics.py:398-418).
+//   k_reduce<M, MODE>  CFL maximum + volume of a state (numerics.py:297-309);
+//                      MODE_INIT also primes the loop control block.
+//   k_cfl_field<M>     per-cell inverse CFL time.
+//   k_sum_mult         FP64 sum of the rain multiplier over active cells.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "swe_device.hpp"
+#include "../../include/swe_hip.h"
+
+namespace swe {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kDefaultRows = 16;
+
+// Device-resident state of the time loop.  The first fields are what every
+// step kernel reads on entry.
+struct LoopControl {
+  float dt_f, dtrate_f, dtg_f;  // float32(dt), float32(dt*rate), float32(dt*g)
+  int done;
+  int rec_lo, rec_hi;  // output slots the pending step writes
+  int steps, out_idx;
+  double t, dt, rate;
+  double vol0, vol, rain, bnd, clip;
+  double max_inv;
+  int nonfinite;
+  unsigned counter;  // workgroups finished in the current launch
+};
+
+struct LoopConst {
+  double cfl, max_dt, gravity, cell_area, sum_mult, rain_default, dxd, dyd;
+  const double* rain_t;
+  const double* rain_r;
+  const double* times;
+  int n_rain, n_times, max_steps, _pad;
+};
+
+struct Partials {
+  float* inv;
+  double* vol;
+  double* clip;
+  double* bnd;
+  int* nf;
+};
+
+enum { MODE_STEP = 0, MODE_LOOP = 1 };
+enum { RED_CFL = 0, RED_INIT = 1 };
+
+template <class T>
+__device__ __forceinline__ T coherent_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Prepare the control block for the step that starts at t_new with the CFL
+// maximum m of the current state (numerics.py:405 with the float64 quotient
+// of numpy_solver/shallow_water_2d.py:274).  Single thread.
+__device__ void plan_next_step(LoopControl* ctl, const LoopConst& L, float m, double t_new,
+                               int oi, int rain_j) {
+  const double md = (double)m;
+  const double cfl_dt = md > 0.0 ? fmin(L.cfl / md, L.max_dt) : L.max_dt;
+  const double target = L.times[oi];
+  const double dt = fmin(cfl_dt, target - t_new);
+  const double t_end = t_new + dt;
+  int hi = oi;
+  while (hi < L.n_times && t_end >= L.times[hi] - kOutputSlack) ++hi;
+  const double rate = interp_at(L.rain_t, L.rain_r, L.n_rain, L.rain_default, t_new, rain_j);
+  ctl->dt = dt;
+  ctl->rate = rate;
+  ctl->dt_f = (float)dt;
+  ctl->dtrate_f = (float)(dt * rate);
+  ctl->dtg_f = (float)(dt * L.gravity);
+  ctl->rec_lo = oi;
+  ctl->rec_hi = hi;
+}
+
+// Block-wide reduction of (max, sum, sum, sum, or) into thread 0.
+struct Totals {
+  float inv;
+  double vol, clip, bnd;
+  int nf;
+};
+
+__device__ Totals block_reduce(float inv, double vol, double clip, double bnd, int nf) {
+  __shared__ float s_inv[kWavesPerBlock];
+  __shared__ double s_vol[kWavesPerBlock], s_clip[kWavesPerBlock], s_bnd[kWavesPerBlock];
+  __shared__ int s_nf[kWavesPerBlock];
+  inv = wave_max(inv);
+  vol = wave_sum(vol);
+  clip = wave_sum(clip);
+  bnd = wave_sum(bnd);
+  nf = wave_or(nf);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_inv[w] = inv;
+    s_vol[w] = vol;
+    s_clip[w] = clip;
+    s_bnd[w] = bnd;
+    s_nf[w] = nf;
+  }
+  __syncthreads();
+  Totals t{0.0f, 0.0, 0.0, 0.0, 0};
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < kWavesPerBlock; ++k) {
+      t.inv = fmaxf(t.inv, s_inv[k]);
+      t.vol += s_vol[k];
+      t.clip += s_clip[k];
+      t.bnd += s_bnd[k];
+      t.nf |= s_nf[k];
+    }
+  }
+  __syncthreads();
+  return t;
+}
+
+// Publish this workgroup's partials; returns true (block-uniform) in the last
+// workgroup of the launch, after which every workgroup's partials are visible
+// to it (release on the counter increment, acquire in the last workgroup).
+__device__ bool publish_partials(const Partials& P, LoopControl* ctl, const Totals& t) {
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    const int b = blockIdx.x;
+    __hip_atomic_store(P.inv + b, t.inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(P.vol + b, t.vol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(P.clip + b, t.clip, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(P.bnd + b, t.bnd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(P.nf + b, t.nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned prev = __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELEASE,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return s_last != 0;
+}
+
+// Sum all partials (last workgroup only) into thread 0.
+__device__ Totals gather_partials(const Partials& P, int nparts) {
+  float inv = 0.0f;
+  double vol = 0.0, clip = 0.0, bnd = 0.0;
+  int nf = 0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    inv = fmaxf(inv, coherent_load(P.inv + k));
+    vol += coherent_load(P.vol + k);
+    clip += coherent_load(P.clip + k);
+    bnd += coherent_load(P.bnd + k);
+    nf |= coherent_load(P.nf + k);
+  }
+  return block_reduce(inv, vol, clip, bnd, nf);
+}
+
+// Index j with rain_t[j] <= t < rain_t[j+1], searched by the whole block.
+__device__ int rain_bracket(const LoopConst& L, double t) {
+  __shared__ int s_j;
+  if (threadIdx.x == 0) s_j = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < L.n_rain - 1; k += blockDim.x) {
+    if (L.rain_t[k] <= t && t < L.rain_t[k + 1]) s_j = k;
+  }
+  __syncthreads();
+  return s_j;
+}
+
+// ---------------------------------------------------------------------------
+// The fused step
+// ---------------------------------------------------------------------------
+template <class M, int MODE>
+__global__ __launch_bounds__(kBlock) void k_step(
+    Grid G, const float* __restrict__ H, const float* __restrict__ HU,
+    const float* __restrict__ HV, float* __restrict__ Ho, float* __restrict__ HUo,
+    float* __restrict__ HVo, float dt_arg, float dtrate_arg, float dtg_arg, int rows,
+    int segs_x, int n_waves, LoopControl* __restrict__ ctl, LoopConst L, Partials P,
+    float* __restrict__ cube_h, float* __restrict__ cube_hu, float* __restrict__ cube_hv) {
+  float dt = dt_arg, dtrate = dtrate_arg, dtg = dtg_arg;
+  int rec_lo = 0, rec_hi = 0;
+  if constexpr (MODE == MODE_LOOP) {
+    if (ctl->done) return;
+    dt = ctl->dt_f;
+    dtrate = ctl->dtrate_f;
+    dtg = ctl->dtg_f;
+    rec_lo = ctl->rec_lo;
+    rec_hi = ctl->rec_hi;
+  }
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int gw = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave);
+
+  float inv_max = 0.0f;
+  double vol = 0.0, clip = 0.0, bnd = 0.0;
+  int nf = 0;
+
+  if (gw < n_waves) {
+    const int sx = gw % segs_x;
+    const int sy = gw / segs_x;
+    const int i = sx * kStripCols - 1 + lane;
+    const bool own = lane >= 1 && lane <= kStripCols && i < G.nx;
+    const int ic = min(max(i, 0), G.nx - 1);
+    const int j0 = sy * rows;
+    const int j1 = min(j0 + rows, G.ny);
+
+    const Cell below = load_cell<M>(G, H, HU, HV, j0 - 1, i);
+    Cell cur = load_cell<M>(G, H, HU, HV, j0, i);
+    FaceFlux fy = face_flux<M, false>(G, below.h, below.v, below.u, below.b, below.a, cur.h,
+                                      cur.v, cur.u, cur.b, cur.a);
+    // South face of the first row, seen from its north (top) side.
+    float yt0 = fy.f0, yt1 = fy.ft, yt2 = fy.fn + fy.cB;
+    if (MODE == MODE_LOOP && j0 == 0 && own) bnd += (double)fy.f0 * L.dxd;
+
+    for (int j = j0; j < j1; ++j) {
+      const Cell up = load_cell<M>(G, H, HU, HV, j + 1, i);
+      const size_t idx = (size_t)j * (size_t)G.nx + (size_t)ic;
+      const float nn = G.n[idx];
+      const float mult = G.mult[idx];
+
+      // West face of this lane's column; the lane to the west supplies its cell.
+      const float wh = from_west(cur.h), wu = from_west(cur.u), wv = from_west(cur.v);
+      const float wb = from_west(cur.b);
+      const int wa = from_west(cur.a);
+      const FaceFlux fx =
+          face_flux<M, true>(G, wh, wu, wv, wb, wa, cur.h, cur.u, cur.v, cur.b, cur.a);
+      // This cell is the east (right) side of its west face ...
+      const float xr0 = fx.f0, xr1 = fx.fn + fx.cB, xr2 = fx.ft;
+      // ... and the west (left) side of its east face, evaluated by lane+1.
+      const float xl0 = from_east(fx.f0);
+      const float xl1 = from_east(fx.fn + fx.cA);
+      const float xl2 = from_east(fx.ft);
+      if (MODE == MODE_LOOP) {
+        if (i == 0) bnd += (double)fx.f0 * L.dyd;
+        if (i == G.nx) bnd -= (double)fx.f0 * L.dyd;
+      }
+
+      // North face: this cell is its south (bottom) side.
+      const FaceFlux gy = face_flux<M, false>(G, cur.h, cur.v, cur.u, cur.b, cur.a, up.h, up.v,
+                                              up.u, up.b, up.a);
+      const float yb0 = gy.f0, yb1 = gy.ft, yb2 = gy.fn + gy.cA;
+      if (MODE == MODE_LOOP && j + 1 == G.ny && own) bnd -= (double)gy.f0 * L.dxd;
+
+      const float d0 = M::per_dx(xl0 - xr0, G.dx, G.rdx) + M::per_dx(yb0 - yt0, G.dy, G.rdy);
+      const float d1 = M::per_dx(xl1 - xr1, G.dx, G.rdx) + M::per_dx(yb1 - yt1, G.dy, G.rdy);
+      const float d2 = M::per_dx(xl2 - xr2, G.dx, G.rdx) + M::per_dx(yb2 - yt2, G.dy, G.rdy);
+      const CellUpdate r =
+          cell_update<M>(G, cur.h, cur.hu, cur.hv, d0, d1, d2, nn, mult, cur.a, dt, dtrate, dtg);
+      if (own) {
+        Ho[idx] = r.h;
+        HUo[idx] = r.hu;
+        HVo[idx] = r.hv;
+        if constexpr (MODE == MODE_LOOP) {
+          for (int k = rec_lo; k < rec_hi; ++k) {
+            cube_h[(size_t)k * N + idx] = r.h;
+            cube_hu[(size_t)k * N + idx] = r.hu;
+            cube_hv[(size_t)k * N + idx] = r.hv;
+          }
+          inv_max = fmaxf(inv_max, cfl_inverse<M>(G, r.h, r.hu, r.hv, cur.a));
+          vol += (double)r.h;
+          if (cur.a) clip += (double)fmaxf(-r.h_star, 0.0f);
+          nf |= !(isfinite(r.h) && isfinite(r.hu) && isfinite(r.hv));
+        }
+      }
+      yt0 = gy.f0;
+      yt1 = gy.ft;
+      yt2 = gy.fn + gy.cB;
+      cur = up;
+    }
+  }
+
+  if constexpr (MODE == MODE_LOOP) {
+    const Totals mine = block_reduce(inv_max, vol, clip, bnd, nf);
+    if (!publish_partials(P, ctl, mine)) return;
+    // Last workgroup: finish the step on the device.
+    const Totals tot = gather_partials(P, gridDim.x);
+    const double t_new = ctl->t + ctl->dt;
+    const int rj = rain_bracket(L, t_new);
+    if (threadIdx.x == 0) {
+      ctl->rain += (double)ctl->dtrate_f * L.sum_mult * L.cell_area;
+      ctl->bnd += ctl->dt * tot.bnd;
+      ctl->clip += tot.clip * L.cell_area;
+      ctl->vol = tot.vol * L.cell_area;
+      ctl->nonfinite |= tot.nf;
+      ctl->max_inv = (double)tot.inv;
+      const int steps = ctl->steps + 1;
+      ctl->steps = steps;
+      ctl->t = t_new;
+      int oi = ctl->out_idx;
+      while (oi < L.n_times && t_new >= L.times[oi] - kOutputSlack) ++oi;
+      ctl->out_idx = oi;
+      if (oi >= L.n_times || steps >= L.max_steps || ctl->nonfinite) {
+        ctl->done = 1;
+        ctl->rec_lo = ctl->rec_hi = 0;
+      } else {
+        plan_next_step(ctl, L, tot.inv, t_new, oi, rj);
+      }
+      __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CFL maximum / volume of a state; MODE RED_INIT primes the loop.
+// ---------------------------------------------------------------------------
+template <class M, int MODE>
+__global__ __launch_bounds__(kBlock) void k_reduce(Grid G, const float* __restrict__ H,
+                                                  const float* __restrict__ HU,
+                                                  const float* __restrict__ HV,
+                                                  LoopControl* __restrict__ ctl, LoopConst L,
+                                                  Partials P) {
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  float inv = 0.0f;
+  double vol = 0.0;
+  int nf = 0;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    const float h = H[k], hu = HU[k], hv = HV[k];
+    const int a = G.mask[k] != 0;
+    inv = fmaxf(inv, cfl_inverse<M>(G, h, hu, hv, a));
+    vol += a ? (double)h : 0.0;
+    nf |= !(isfinite(h) && isfinite(hu) && isfinite(hv));
+  }
+  const Totals mine = block_reduce(inv, vol, 0.0, 0.0, nf);
+  if (!publish_partials(P, ctl, mine)) return;
+  const Totals tot = gather_partials(P, gridDim.x);
+  double t0 = 0.0;
+  if (MODE == RED_INIT) t0 = L.times[0];
+  const int rj = rain_bracket(L, t0);
+  if (threadIdx.x == 0) {
+    ctl->max_inv = (double)tot.inv;
+    if (MODE == RED_INIT) {
+      ctl->t = t0;
+      ctl->dt = 0.0;
+      ctl->steps = 0;
+      ctl->vol0 = ctl->vol = tot.vol * L.cell_area;
+      ctl->rain = ctl->bnd = ctl->clip = 0.0;
+      ctl->nonfinite = tot.nf;
+      int oi = 1;
+      while (oi < L.n_times && t0 >= L.times[oi] - kOutputSlack) ++oi;
+      ctl->out_idx = oi;
+      if (oi >= L.n_times || L.max_steps <= 0 || tot.nf) {
+        ctl->done = 1;
+        ctl->rec_lo = ctl->rec_hi = 0;
+      } else {
+        ctl->done = 0;
+        plan_next_step(ctl, L, tot.inv, t0, oi, rj);
+      }
+    }
+    __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <class M>
+__global__ __launch_bounds__(kBlock) void k_cfl_field(Grid G, const float* __restrict__ H,
+                                                     const float* __restrict__ HU,
+                                                     const float* __restrict__ HV,
+                                                     float* __restrict__ out) {
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    out[k] = cfl_inverse<M>(G, H[k], HU[k], HV[k], G.mask[k] != 0);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_mult(Grid G, double* __restrict__ partial) {
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  double s = 0.0;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    if (G.mask[k]) s += (double)G.mult[k];
+  }
+  const Totals t = block_reduce(0.0f, s, 0.0, 0.0, 0);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t.vol;
+}
+
+// Copy the loop's final state (buffer A or B by the parity of the step count)
+// to `dst`, so the host need not know how many steps the device took.
+__global__ __launch_bounds__(kBlock) void k_copy_final(const LoopControl* __restrict__ ctl,
+                                                      const float* __restrict__ a,
+                                                      const float* __restrict__ b,
+                                                      float* __restrict__ d0,
+                                                      float* __restrict__ d1,
+                                                      float* __restrict__ d2, size_t N) {
+  const float* src = (ctl->steps & 1) ? b : a;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    d0[k] = src[k];
+    d1[k] = src[N + k];
+    d2[k] = src[2 * N + k];
+  }
+}
+
+}  // namespace swe
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+using namespace swe;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define SWE_HIP_TRY(expr)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(SWE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+}  // namespace
+
+struct SweContext {
+  int device = 0;
+  Grid G{};
+  int variant = SWE_VARIANT_FAST;
+  int rows = kDefaultRows;
+  int graph_chunk = 64;
+  double dx = 1, dy = 1, gravity = 9.81, cfl = 0.45, dry = 1e-4, max_dt = 10, rain_default = 0;
+  std::vector<double> rain_t, rain_r;
+  double sum_mult = 0.0;
+  double* d_rain = nullptr;  // [rain_t | rain_r]
+  double* d_times = nullptr;
+  int times_cap = 0;
+  LoopControl* d_ctl = nullptr;
+  LoopControl* h_ctl = nullptr;  // pinned mirror
+  Partials P{};
+  int parts_cap = 0;
+  float* work = nullptr;  // 2 x 3 x N ping-pong state
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  std::vector<const void*> graph_key;
+  // swe_loop_begin/advance/end state
+  double loop_times[2] = {0.0, 0.0};
+  LoopConst loop_L{};
+  bool loop_active = false;
+
+  size_t cells() const { return (size_t)G.nx * (size_t)G.ny; }
+  float* buf(int which, int field) { return work + ((size_t)which * 3 + field) * cells(); }
+  int segs_x() const { return (G.nx + kStripCols - 1) / kStripCols; }
+  int n_waves() const { return segs_x() * ((G.ny + rows - 1) / rows); }
+  int step_blocks() const { return (n_waves() + kWavesPerBlock - 1) / kWavesPerBlock; }
+  int reduce_blocks() const {
+    const size_t b = (cells() + (size_t)kBlock * 8 - 1) / ((size_t)kBlock * 8);
+    return (int)std::max<size_t>(1, std::min<size_t>(b, 2048));
+  }
+  LoopConst loop_const(int n_times, int max_steps) const {
+    LoopConst L{};
+    L.cfl = cfl;
+    L.max_dt = max_dt;
+    L.gravity = gravity;
+    L.cell_area = dx * dy;
+    L.sum_mult = sum_mult;
+    L.rain_default = rain_default;
+    L.dxd = dx;
+    L.dyd = dy;
+    L.rain_t = d_rain;
+    L.rain_r = d_rain + rain_t.size();
+    L.times = d_times;
+    L.n_rain = (int)rain_t.size();
+    L.n_times = n_times;
+    L.max_steps = max_steps;
+    return L;
+  }
+};
+
+namespace {
+
+int ensure_partials(SweContext* c) {
+  const int need = std::max(c->step_blocks(), c->reduce_blocks());
+  if (need <= c->parts_cap) return SWE_OK;
+  if (c->P.vol) {
+    SWE_HIP_TRY(hipFree(c->P.vol));
+    c->P = Partials{};
+    c->parts_cap = 0;
+  }
+  // One allocation: vol | clip | bnd (double) | inv (float) | nf (int).
+  const size_t bytes = (size_t)need * (3 * sizeof(double) + sizeof(float) + sizeof(int));
+  char* base = nullptr;
+  SWE_HIP_TRY(hipMalloc(&base, bytes));
+  c->P.vol = reinterpret_cast<double*>(base);
+  c->P.clip = c->P.vol + need;
+  c->P.bnd = c->P.clip + need;
+  c->P.inv = reinterpret_cast<float*>(c->P.bnd + need);
+  c->P.nf = reinterpret_cast<int*>(c->P.inv + need);
+  c->parts_cap = need;
+  return SWE_OK;
+}
+
+int ensure_times(SweContext* c, int n) {
+  if (n <= c->times_cap) return SWE_OK;
+  if (c->d_times) SWE_HIP_TRY(hipFree(c->d_times));
+  c->d_times = nullptr;
+  SWE_HIP_TRY(hipMalloc(&c->d_times, sizeof(double) * (size_t)n));
+  c->times_cap = n;
+  return SWE_OK;
+}
+
+int ensure_work(SweContext* c) {
+  if (c->work) return SWE_OK;
+  SWE_HIP_TRY(hipMalloc(&c->work, sizeof(float) * 6 * c->cells()));
+  return SWE_OK;
+}
+
+bool valid_state(const SweState& s) { return s.h && s.hu && s.hv; }
+
+template <class M, int MODE>
+void launch_step(SweContext* c, const float* h, const float* hu, const float* hv, float* ho,
+                 float* huo, float* hvo, float dt, float dtrate, float dtg, const LoopConst& L,
+                 float* ch, float* chu, float* chv, hipStream_t s) {
+  hipLaunchKernelGGL((k_step<M, MODE>), dim3(c->step_blocks()), dim3(kBlock), 0, s, c->G, h, hu,
+                     hv, ho, huo, hvo, dt, dtrate, dtg, c->rows, c->segs_x(), c->n_waves(),
+                     c->d_ctl, L, c->P, ch, chu, chv);
+}
+
+void launch_step_any(SweContext* c, int mode, const float* h, const float* hu, const float* hv,
+                     float* ho, float* huo, float* hvo, float dt, float dtrate, float dtg,
+                     const LoopConst& L, float* ch, float* chu, float* chv, hipStream_t s) {
+  if (c->variant == SWE_VARIANT_REFERENCE) {
+    if (mode == MODE_LOOP)
+      launch_step<RefMath, MODE_LOOP>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch, chu,
+                                      chv, s);
+    else
+      launch_step<RefMath, MODE_STEP>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch, chu,
+                                      chv, s);
+  } else {
+    if (mode == MODE_LOOP)
+      launch_step<FastMath, MODE_LOOP>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch, chu,
+                                       chv, s);
+    else
+      launch_step<FastMath, MODE_STEP>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch, chu,
+                                       chv, s);
+  }
+}
+
+void launch_reduce(SweContext* c, int mode, const float* h, const float* hu, const float* hv,
+                   const LoopConst& L, hipStream_t s) {
+  const dim3 grid(c->reduce_blocks()), block(kBlock);
+  if (c->variant == SWE_VARIANT_REFERENCE) {
+    if (mode == RED_INIT)
+      hipLaunchKernelGGL((k_reduce<RefMath, RED_INIT>), grid, block, 0, s, c->G, h, hu, hv,
+                         c->d_ctl, L, c->P);
+    else
+      hipLaunchKernelGGL((k_reduce<RefMath, RED_CFL>), grid, block, 0, s, c->G, h, hu, hv,
+                         c->d_ctl, L, c->P);
+  } else {
+    if (mode == RED_INIT)
+      hipLaunchKernelGGL((k_reduce<FastMath, RED_INIT>), grid, block, 0, s, c->G, h, hu, hv,
+                         c->d_ctl, L, c->P);
+    else
+      hipLaunchKernelGGL((k_reduce<FastMath, RED_CFL>), grid, block, 0, s, c->G, h, hu, hv,
+                         c->d_ctl, L, c->P);
+  }
+}
+
+double host_interp(const SweContext* c, double t) {
+  const int n = (int)c->rain_t.size();
+  int j = 0;
+  if (n > 1) {
+    // largest j with x[j] <= t (only meaningful inside [x0, x_{n-1}))
+    const auto it = std::upper_bound(c->rain_t.begin(), c->rain_t.end(), t);
+    j = (int)(it - c->rain_t.begin()) - 1;
+    j = std::min(std::max(j, 0), n - 2);
+  }
+  return interp_at(c->rain_t.data(), c->rain_r.data(), n, c->rain_default, t, j);
+}
+
+// Launch `count` loop steps starting at launch parity 0 (buffer A -> B first).
+int launch_loop_chunk(SweContext* c, int count, const LoopConst& L, float* ch, float* chu,
+                      float* chv, hipStream_t s) {
+  if (c->graph_chunk > 0) {
+    std::vector<const void*> key = {c->work, ch, chu, chv, (const void*)(intptr_t)c->variant,
+                                    (const void*)(intptr_t)c->rows,
+                                    (const void*)(intptr_t)c->graph_chunk, c->d_times,
+                                    (const void*)(intptr_t)L.n_times,
+                                    (const void*)(intptr_t)L.max_steps};
+    if (!c->graph || key != c->graph_key) {
+      if (c->graph) {
+        SWE_HIP_TRY(hipGraphExecDestroy(c->graph));
+        c->graph = nullptr;
+      }
+      if (!c->cap_stream) SWE_HIP_TRY(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+      SWE_HIP_TRY(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < c->graph_chunk; ++k) {
+        const int a = k & 1;
+        launch_step_any(c, MODE_LOOP, c->buf(a, 0), c->buf(a, 1), c->buf(a, 2), c->buf(1 - a, 0),
+                        c->buf(1 - a, 1), c->buf(1 - a, 2), 0.f, 0.f, 0.f, L, ch, chu, chv,
+                        c->cap_stream);
+      }
+      hipGraph_t g = nullptr;
+      SWE_HIP_TRY(hipStreamEndCapture(c->cap_stream, &g));
+      hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(SWE_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      c->graph_key = key;
+    }
+    for (int done = 0; done < count; done += c->graph_chunk) SWE_HIP_TRY(hipGraphLaunch(c->graph, s));
+    return SWE_OK;
+  }
+  for (int k = 0; k < count; ++k) {
+    const int a = k & 1;
+    launch_step_any(c, MODE_LOOP, c->buf(a, 0), c->buf(a, 1), c->buf(a, 2), c->buf(1 - a, 0),
+                    c->buf(1 - a, 1), c->buf(1 - a, 2), 0.f, 0.f, 0.f, L, ch, chu, chv, s);
+  }
+  SWE_HIP_TRY(hipGetLastError());
+  return SWE_OK;
+}
+
+// Run the device loop until the control block says done.  Chunks of steps
+// are queued two deep; after each chunk the control block is copied to a
+// pinned slot and the older chunk's copy decides whether to continue.  On
+// return c->h_ctl[0] holds the final control block.
+int drive_loop(SweContext* c, const LoopConst& L, float* ch, float* chu, float* chv,
+               hipStream_t s, int max_steps) {
+  const int chunk = c->graph_chunk > 0 ? c->graph_chunk : 64;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = SWE_OK;
+  LoopControl* hc = c->h_ctl;
+  long launched = 0;
+  bool have_prev = false;
+  int slot = 0;
+  hipError_t e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+  if (e != hipSuccess) rc = fail(SWE_EHIP, std::string("loop events: ") + hipGetErrorString(e));
+  while (rc == SWE_OK) {
+    rc = launch_loop_chunk(c, chunk, L, ch, chu, chv, s);
+    if (rc) break;
+    launched += chunk;
+    e = hipMemcpyAsync(hc + slot, c->d_ctl, sizeof(LoopControl), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[slot], s);
+    if (e != hipSuccess) {
+      rc = fail(SWE_EHIP, std::string("loop poll: ") + hipGetErrorString(e));
+      break;
+    }
+    if (have_prev) {
+      const int prev = 1 - slot;
+      e = hipEventSynchronize(ev[prev]);
+      if (e != hipSuccess) {
+        rc = fail(SWE_EHIP, std::string("loop sync: ") + hipGetErrorString(e));
+        break;
+      }
+      if (hc[prev].done) break;
+    }
+    have_prev = true;
+    slot = 1 - slot;
+    // The device stops itself at max_steps; this bound only guards the host.
+    if (launched > (long)max_steps + 4L * chunk) {
+      rc = fail(SWE_EHIP, "device loop did not stop at max_steps");
+      break;
+    }
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess && rc == SWE_OK) rc = fail(SWE_EHIP, std::string("loop end: ") + hipGetErrorString(e));
+  if (rc == SWE_OK) {
+    e = hipMemcpy(hc, c->d_ctl, sizeof(LoopControl), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(SWE_EHIP, std::string("loop read: ") + hipGetErrorString(e));
+  }
+  if (ev[0]) (void)hipEventDestroy(ev[0]);
+  if (ev[1]) (void)hipEventDestroy(ev[1]);
+  return rc;
+}
+
+int copy_final(SweContext* c, SweState dst, hipStream_t s) {
+  const size_t N = c->cells();
+  const int nb = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 4096);
+  hipLaunchKernelGGL(k_copy_final, dim3(nb), dim3(kBlock), 0, s, c->d_ctl, c->buf(0, 0),
+                     c->buf(1, 0), dst.h, dst.hu, dst.hv, N);
+  SWE_HIP_TRY(hipGetLastError());
+  return SWE_OK;
+}
+
+void fill_info(const SweContext* c, const LoopControl& k, SweRunInfo* info) {
+  (void)c;
+  if (!info) return;
+  info->outputs_written = k.out_idx;
+  info->steps_used = k.steps;
+  info->final_time_s = k.t;
+  info->volume_initial = k.vol0;
+  info->volume_final = k.vol;
+  info->rain_volume = k.rain;
+  info->boundary_volume = k.bnd;
+  info->clip_volume = k.clip;
+  info->mass_balance_error = k.vol - k.vol0 - k.rain - k.bnd - k.clip;
+  const double scale = std::max(std::fabs(k.vol0) + std::fabs(k.rain) + std::fabs(k.bnd) +
+                                    std::fabs(k.clip),
+                                1.0);
+  info->mass_balance_rel = info->mass_balance_error / scale;
+  info->nonfinite = k.nonfinite;
+  info->_pad = 0;
+}
+
+int copy_state(SweContext* c, float* dh, float* dhu, float* dhv, const float* sh,
+               const float* shu, const float* shv, hipStream_t s) {
+  const size_t bytes = sizeof(float) * c->cells();
+  SWE_HIP_TRY(hipMemcpyAsync(dh, sh, bytes, hipMemcpyDeviceToDevice, s));
+  SWE_HIP_TRY(hipMemcpyAsync(dhu, shu, bytes, hipMemcpyDeviceToDevice, s));
+  SWE_HIP_TRY(hipMemcpyAsync(dhv, shv, bytes, hipMemcpyDeviceToDevice, s));
+  return SWE_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* swe_version(void) { return "swe_hip 0.1 (gfx950)"; }
+const char* swe_last_error(void) { return g_last_error.c_str(); }
+
+int swe_interp_rate(const float* times_s, const float* rates_m_s, int32_t n, double dflt,
+                    double time_s, double* rate_out) {
+  if (!times_s || !rates_m_s || n < 1 || !rate_out) return fail(SWE_EINVAL, "swe_interp_rate: bad arguments");
+  std::vector<double> x(times_s, times_s + n), y(rates_m_s, rates_m_s + n);
+  int j = 0;
+  if (n > 1) {
+    const auto it = std::upper_bound(x.begin(), x.end(), time_s);
+    j = std::min(std::max((int)(it - x.begin()) - 1, 0), n - 2);
+  }
+  *rate_out = interp_at(x.data(), y.data(), n, dflt, time_s, j);
+  return SWE_OK;
+}
+
+int swe_create(const SweParamsDesc* p, int32_t variant, SweContext** out) {
+  if (!p || !out) return fail(SWE_EINVAL, "swe_create: null argument");
+  *out = nullptr;
+  if (p->nx < 1 || p->ny < 1) return fail(SWE_EINVAL, "swe_create: grid must be at least 1x1");
+  if ((int64_t)p->nx * (int64_t)p->ny > (int64_t)1 << 31)
+    return fail(SWE_EINVAL, "swe_create: grid larger than 2^31 cells");
+  if (!p->bed || !p->manning_n || !p->domain_mask || !p->rainfall_multiplier)
+    return fail(SWE_EINVAL, "swe_create: a field pointer is null");
+  if (p->n_rainfall < 1 || !p->rainfall_times_s || !p->rainfall_rates_m_s)
+    return fail(SWE_EINVAL, "swe_create: at least one rainfall sample is required");
+  for (int k = 1; k < p->n_rainfall; ++k)
+    if (!(p->rainfall_times_s[k] > p->rainfall_times_s[k - 1]))
+      return fail(SWE_EINVAL, "swe_create: rainfall times must increase strictly");
+  for (int k = 0; k < 4; ++k)
+    if (p->boundary_types[k] != SWE_REFLECTIVE && p->boundary_types[k] != SWE_OPEN)
+      return fail(SWE_EINVAL, "swe_create: boundary types must be 0 (reflective) or 1 (open)");
+  if (!(p->dx > 0) || !(p->dy > 0) || !(p->gravity > 0) || !(p->cfl > 0) ||
+      !(p->dry_tolerance > 0) || !(p->max_dt > 0))
+    return fail(SWE_EINVAL, "swe_create: dx, dy, gravity, cfl, dry_tolerance, max_dt must be > 0");
+  if (variant != SWE_VARIANT_REFERENCE && variant != SWE_VARIANT_FAST)
+    return fail(SWE_EINVAL, "swe_create: unknown kernel variant");
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(SWE_ENODEV, "swe_create: no HIP device is visible");
+  int dev = 0;
+  SWE_HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  SWE_HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(SWE_ENODEV, std::string("swe_create: device is ") + prop.gcnArchName +
+                                ", this library is built for gfx950");
+
+  SweContext* c = new SweContext();
+  c->device = dev;
+  c->variant = variant;
+  c->dx = p->dx;
+  c->dy = p->dy;
+  c->gravity = p->gravity;
+  c->cfl = p->cfl;
+  c->dry = p->dry_tolerance;
+  c->max_dt = p->max_dt;
+  c->rain_default = p->default_rainfall_m_s;
+  c->rain_t.assign(p->rainfall_times_s, p->rainfall_times_s + p->n_rainfall);
+  c->rain_r.assign(p->rainfall_rates_m_s, p->rainfall_rates_m_s + p->n_rainfall);
+  Grid& G = c->G;
+  G.nx = p->nx;
+  G.ny = p->ny;
+  for (int k = 0; k < 4; ++k) G.bt[k] = p->boundary_types[k];
+  // Array-side scalars are float32 (jax_solver/shallow_water_2d.py:246-251);
+  // 0.5*g is the float64 product rounded once (numpy_solver/shallow_water_2d.py:84).
+  G.g = (float)p->gravity;
+  G.half_g = (float)(0.5 * p->gravity);
+  G.dry = (float)p->dry_tolerance;
+  G.dx = (float)p->dx;
+  G.dy = (float)p->dy;
+  G.rdx = 1.0f / G.dx;
+  G.rdy = 1.0f / G.dy;
+  G.bed = p->bed;
+  G.n = p->manning_n;
+  G.mask = p->domain_mask;
+  G.mult = p->rainfall_multiplier;
+
+  auto cleanup = [&](int rc) {
+    swe_destroy(c);
+    return rc;
+  };
+  if (hipMalloc(&c->d_rain, sizeof(double) * 2 * c->rain_t.size()) != hipSuccess)
+    return cleanup(fail(SWE_EHIP, "swe_create: hipMalloc rain"));
+  std::vector<double> rr(c->rain_t);
+  rr.insert(rr.end(), c->rain_r.begin(), c->rain_r.end());
+  if (hipMemcpy(c->d_rain, rr.data(), sizeof(double) * rr.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(SWE_EHIP, "swe_create: upload rain"));
+  if (hipMalloc(&c->d_ctl, sizeof(LoopControl)) != hipSuccess)
+    return cleanup(fail(SWE_EHIP, "swe_create: hipMalloc control"));
+  if (hipMemset(c->d_ctl, 0, sizeof(LoopControl)) != hipSuccess)
+    return cleanup(fail(SWE_EHIP, "swe_create: clear control"));
+  if (hipHostMalloc(&c->h_ctl, 2 * sizeof(LoopControl), hipHostMallocDefault) != hipSuccess)
+    return cleanup(fail(SWE_EHIP, "swe_create: hipHostMalloc control"));
+  int rc = ensure_partials(c);
+  if (rc) return cleanup(rc);
+  // Sum of the rain multiplier over active cells, for the FP64 rain volume.
+  {
+    const int nb = c->reduce_blocks();
+    hipLaunchKernelGGL(k_sum_mult, dim3(nb), dim3(kBlock), 0, 0, c->G, c->P.vol);
+    std::vector<double> part(nb);
+    if (hipMemcpy(part.data(), c->P.vol, sizeof(double) * nb, hipMemcpyDeviceToHost) != hipSuccess)
+      return cleanup(fail(SWE_EHIP, "swe_create: rain multiplier sum"));
+    double s = 0.0;
+    for (double v : part) s += v;
+    c->sum_mult = s;
+  }
+  *out = c;
+  return SWE_OK;
+}
+
+int swe_destroy(SweContext* c) {
+  if (!c) return SWE_OK;
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
+  if (c->work) (void)hipFree(c->work);
+  if (c->P.vol) (void)hipFree(c->P.vol);
+  if (c->d_times) (void)hipFree(c->d_times);
+  if (c->d_ctl) (void)hipFree(c->d_ctl);
+  if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+  if (c->d_rain) (void)hipFree(c->d_rain);
+  delete c;
+  return SWE_OK;
+}
+
+int swe_set_variant(SweContext* c, int32_t variant) {
+  if (!c) return fail(SWE_EINVAL, "swe_set_variant: null context");
+  if (variant != SWE_VARIANT_REFERENCE && variant != SWE_VARIANT_FAST)
+    return fail(SWE_EINVAL, "swe_set_variant: unknown kernel variant");
+  c->variant = variant;
+  return SWE_OK;
+}
+
+int swe_set_tile_rows(SweContext* c, int32_t rows) {
+  if (!c) return fail(SWE_EINVAL, "swe_set_tile_rows: null context");
+  if (rows < 0 || rows > 4096) return fail(SWE_EINVAL, "swe_set_tile_rows: rows must be in [0, 4096]");
+  c->rows = rows == 0 ? kDefaultRows : rows;
+  return ensure_partials(c);
+}
+
+int swe_set_graph_chunk(SweContext* c, int32_t chunk) {
+  if (!c) return fail(SWE_EINVAL, "swe_set_graph_chunk: null context");
+  if (chunk < 0 || chunk > 4096 || (chunk & 1))
+    return fail(SWE_EINVAL, "swe_set_graph_chunk: chunk must be an even number in [0, 4096]");
+  c->graph_chunk = chunk;
+  return SWE_OK;
+}
+
+int swe_rainfall_rate(const SweContext* c, double time_s, double* rate_out) {
+  if (!c || !rate_out) return fail(SWE_EINVAL, "swe_rainfall_rate: null argument");
+  *rate_out = host_interp(c, time_s);
+  return SWE_OK;
+}
+
+int swe_step(SweContext* c, SweState in, SweState out, double dt, double time_s, void* stream) {
+  if (!c) return fail(SWE_EINVAL, "swe_step: null context");
+  if (!valid_state(in) || !valid_state(out)) return fail(SWE_EINVAL, "swe_step: null state pointer");
+  if (in.h == out.h || in.hu == out.hu || in.hv == out.hv || in.h == out.hu || in.h == out.hv ||
+      in.hu == out.h || in.hu == out.hv || in.hv == out.h || in.hv == out.hu)
+    return fail(SWE_EINVAL, "swe_step: input and output state must not alias");
+  const double rate = host_interp(c, time_s);
+  const float dt_f = (float)dt, dtrate_f = (float)(dt * rate), dtg_f = (float)(dt * c->gravity);
+  const LoopConst L = c->loop_const(0, 0);
+  launch_step_any(c, MODE_STEP, in.h, in.hu, in.hv, out.h, out.hu, out.hv, dt_f, dtrate_f, dtg_f,
+                  L, nullptr, nullptr, nullptr, (hipStream_t)stream);
+  SWE_HIP_TRY(hipGetLastError());
+  return SWE_OK;
+}
+
+int swe_cfl_field(SweContext* c, SweState in, float* inv_out, void* stream) {
+  if (!c || !valid_state(in) || !inv_out) return fail(SWE_EINVAL, "swe_cfl_field: null argument");
+  const size_t N = c->cells();
+  const int nb = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 4096);
+  if (c->variant == SWE_VARIANT_REFERENCE)
+    hipLaunchKernelGGL(k_cfl_field<RefMath>, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, c->G,
+                       in.h, in.hu, in.hv, inv_out);
+  else
+    hipLaunchKernelGGL(k_cfl_field<FastMath>, dim3(nb), dim3(kBlock), 0, (hipStream_t)stream, c->G,
+                       in.h, in.hu, in.hv, inv_out);
+  SWE_HIP_TRY(hipGetLastError());
+  return SWE_OK;
+}
+
+int swe_cfl_dt(SweContext* c, SweState in, double* dt_out, void* stream) {
+  if (!c || !valid_state(in) || !dt_out) return fail(SWE_EINVAL, "swe_cfl_dt: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ensure_partials(c);
+  if (rc) return rc;
+  launch_reduce(c, RED_CFL, in.h, in.hu, in.hv, c->loop_const(0, 0), s);
+  SWE_HIP_TRY(hipGetLastError());
+  SWE_HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopControl), hipMemcpyDeviceToHost, s));
+  SWE_HIP_TRY(hipStreamSynchronize(s));
+  const double m = c->h_ctl->max_inv;
+  *dt_out = m > 0.0 ? std::min(c->cfl / m, c->max_dt) : c->max_dt;
+  return SWE_OK;
+}
+
+int swe_run_to_outputs(SweContext* c, SweState initial, const double* times, int32_t n_times,
+                       int32_t max_steps, float* out_h, float* out_hu, float* out_hv,
+                       SweState final_state, SweRunInfo* info, void* stream) {
+  if (!c || !valid_state(initial)) return fail(SWE_EINVAL, "swe_run_to_outputs: null argument");
+  if (!times || n_times < 1)
+    return fail(SWE_EINVAL, "At least one one-dimensional output time is required.");
+  for (int k = 0; k < n_times; ++k)
+    if (!std::isfinite(times[k])) return fail(SWE_EINVAL, "Output times must be finite.");
+  if (!out_h || !out_hu || !out_hv) return fail(SWE_EINVAL, "swe_run_to_outputs: null output cube");
+  if (max_steps < 0) return fail(SWE_EINVAL, "swe_run_to_outputs: max_steps must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  if ((rc = ensure_work(c)) || (rc = ensure_partials(c)) || (rc = ensure_times(c, n_times))) return rc;
+  SWE_HIP_TRY(hipMemcpyAsync(c->d_times, times, sizeof(double) * n_times, hipMemcpyHostToDevice, s));
+  const size_t N = c->cells();
+  if ((rc = copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), initial.h, initial.hu,
+                       initial.hv, s)))
+    return rc;
+  if ((rc = copy_state(c, out_h, out_hu, out_hv, initial.h, initial.hu, initial.hv, s))) return rc;
+  const LoopConst L = c->loop_const(n_times, max_steps);
+  launch_reduce(c, RED_INIT, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), L, s);
+  SWE_HIP_TRY(hipGetLastError());
+  // Targets already met at t0 hold the initial state (no step is taken for them).
+  int pre = 1;
+  while (pre < n_times && times[0] >= times[pre] - kOutputSlack) {
+    if ((rc = copy_state(c, out_h + (size_t)pre * N, out_hu + (size_t)pre * N,
+                         out_hv + (size_t)pre * N, initial.h, initial.hu, initial.hv, s)))
+      return rc;
+    ++pre;
+  }
+  if ((rc = drive_loop(c, L, out_h, out_hu, out_hv, s, max_steps))) return rc;
+  const LoopControl k = c->h_ctl[0];
+  const int fin = k.steps & 1;  // launch n reads buffer n&1 and writes the other
+  float* fh = c->buf(fin, 0);
+  float* fhu = c->buf(fin, 1);
+  float* fhv = c->buf(fin, 2);
+  for (int slot = k.out_idx; slot < n_times; ++slot) {
+    if ((rc = copy_state(c, out_h + (size_t)slot * N, out_hu + (size_t)slot * N,
+                         out_hv + (size_t)slot * N, fh, fhu, fhv, s)))
+      return rc;
+  }
+  if (valid_state(final_state)) {
+    if ((rc = copy_state(c, final_state.h, final_state.hu, final_state.hv, fh, fhu, fhv, s))) return rc;
+  }
+  SWE_HIP_TRY(hipStreamSynchronize(s));
+  fill_info(c, k, info);
+  if (k.nonfinite) return fail(SWE_ENONFINITE, "a non-finite value appeared in the state");
+  if (k.out_idx != n_times) {
+    char msg[256];
+    std::snprintf(msg, sizeof msg,
+                  "max_steps=%d was insufficient: wrote %d/%d outputs and reached t=%.3fs.",
+                  max_steps, k.out_idx, n_times, k.t);
+    return fail(SWE_EMAXSTEPS, msg);
+  }
+  return SWE_OK;
+}
+
+int swe_loop_begin(SweContext* c, SweState state, double t0, int32_t max_steps, void* stream) {
+  if (!c || !valid_state(state)) return fail(SWE_EINVAL, "swe_loop_begin: null argument");
+  if (max_steps < 0) return fail(SWE_EINVAL, "swe_loop_begin: max_steps must be >= 0");
+  if (!std::isfinite(t0)) return fail(SWE_EINVAL, "swe_loop_begin: t0 must be finite");
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  if ((rc = ensure_work(c)) || (rc = ensure_partials(c)) || (rc = ensure_times(c, 2))) return rc;
+  // No output target: the second "output time" is +inf, so dt = CFL dt.
+  c->loop_times[0] = t0;
+  c->loop_times[1] = HUGE_VAL;
+  SWE_HIP_TRY(hipMemcpyAsync(c->d_times, c->loop_times, sizeof c->loop_times,
+                             hipMemcpyHostToDevice, s));
+  if ((rc = copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), state.h, state.hu, state.hv, s)))
+    return rc;
+  c->loop_L = c->loop_const(2, max_steps);
+  launch_reduce(c, RED_INIT, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), c->loop_L, s);
+  SWE_HIP_TRY(hipGetLastError());
+  c->loop_active = true;
+  return SWE_OK;
+}
+
+int swe_loop_advance(SweContext* c, int32_t n_launches, void* stream) {
+  if (!c || !c->loop_active) return fail(SWE_EINVAL, "swe_loop_advance: call swe_loop_begin first");
+  if (n_launches < 0) return fail(SWE_EINVAL, "swe_loop_advance: n_launches must be >= 0");
+  if (n_launches == 0) return SWE_OK;
+  const int chunk = c->graph_chunk > 0 ? c->graph_chunk : 2;
+  const int launches = ((n_launches + chunk - 1) / chunk) * chunk;
+  return launch_loop_chunk(c, launches, c->loop_L, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int swe_loop_end(SweContext* c, SweState state, SweRunInfo* info, void* stream) {
+  if (!c || !c->loop_active || !valid_state(state))
+    return fail(SWE_EINVAL, "swe_loop_end: call swe_loop_begin first");
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  if ((rc = copy_final(c, state, s))) return rc;
+  if (info) {
+    SWE_HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopControl), hipMemcpyDeviceToHost, s));
+    SWE_HIP_TRY(hipStreamSynchronize(s));
+    fill_info(c, c->h_ctl[0], info);
+    if (c->h_ctl[0].nonfinite) return fail(SWE_ENONFINITE, "a non-finite value appeared in the state");
+  }
+  return SWE_OK;
+}
+
+int swe_run_steps(SweContext* c, SweState state, int32_t n_steps, double t0, SweRunInfo* info,
+                  void* stream) {
+  int rc;
+  if ((rc = swe_loop_begin(c, state, t0, n_steps, stream))) return rc;
+  if ((rc = swe_loop_advance(c, n_steps, stream))) return rc;
+  return swe_loop_end(c, state, info, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+"""ctypes binding of libswe_hip.so (the C ABI in include/swe_hip.h).
+
+The library is built in-tree by ``make`` (or ``__graft_entry__.build()``) and
+loaded from this directory.  torch is imported first so that the library binds
+to the HIP runtime torch already loaded (both carry the soname
+``libamdhip64.so.7``), which lets torch streams and allocations be passed
+straight through.  There is no CPU fallback: if the library is missing or no
+gfx950 device is usable, calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_void_p
+
+import torch  # noqa: F401  (load torch's HIP runtime before the library)
+
+LIB_NAME = "libswe_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+SWE_OK = 0
+SWE_EINVAL = 1
+SWE_EMAXSTEPS = 2
+SWE_EHIP = 3
+SWE_ENONFINITE = 4
+SWE_ENODEV = 5
+
+SWE_VARIANT_REFERENCE = 0
+SWE_VARIANT_FAST = 1
+VARIANTS = {"reference": SWE_VARIANT_REFERENCE, "fast": SWE_VARIANT_FAST}
+
+
+class SweState(ctypes.Structure):
+    _fields_ = [("h", c_void_p), ("hu", c_void_p), ("hv", c_void_p)]
+
+
+class SweParamsDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", c_int32),
+        ("ny", c_int32),
+        ("bed", c_void_p),
+        ("manning_n", c_void_p),
+        ("domain_mask", c_void_p),
+        ("rainfall_multiplier", c_void_p),
+        ("boundary_types", c_int32 * 4),
+        ("rainfall_times_s", POINTER(c_float)),
+        ("rainfall_rates_m_s", POINTER(c_float)),
+        ("n_rainfall", c_int32),
+        ("default_rainfall_m_s", c_double),
+        ("dx", c_double),
+        ("dy", c_double),
+        ("gravity", c_double),
+        ("cfl", c_double),
+        ("dry_tolerance", c_double),
+        ("max_dt", c_double),
+    ]
+
+
+class SweRunInfo(ctypes.Structure):
+    _fields_ = [
+        ("outputs_written", c_int32),
+        ("steps_used", c_int32),
+        ("final_time_s", c_double),
+        ("volume_initial", c_double),
+        ("volume_final", c_double),
+        ("rain_volume", c_double),
+        ("boundary_volume", c_double),
+        ("clip_volume", c_double),
+        ("mass_balance_error", c_double),
+        ("mass_balance_rel", c_double),
+        ("nonfinite", c_int32),
+        ("_pad", c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "_pad"}
+
+
+# name -> (restype, argtypes); every symbol declared in include/swe_hip.h
+SIGNATURES = {
+    "swe_version": (c_char_p, []),
+    "swe_last_error": (c_char_p, []),
+    "swe_create": (c_int32, [POINTER(SweParamsDesc), c_int32, POINTER(c_void_p)]),
+    "swe_destroy": (c_int32, [c_void_p]),
+    "swe_set_variant": (c_int32, [c_void_p, c_int32]),
+    "swe_set_tile_rows": (c_int32, [c_void_p, c_int32]),
+    "swe_set_graph_chunk": (c_int32, [c_void_p, c_int32]),
+    "swe_step": (c_int32, [c_void_p, SweState, SweState, c_double, c_double, c_void_p]),
+    "swe_cfl_dt": (c_int32, [c_void_p, SweState, POINTER(c_double), c_void_p]),
+    "swe_cfl_field": (c_int32, [c_void_p, SweState, c_void_p, c_void_p]),
+    "swe_rainfall_rate": (c_int32, [c_void_p, c_double, POINTER(c_double)]),
+    "swe_interp_rate": (
+        c_int32,
+        [POINTER(c_float), POINTER(c_float), c_int32, c_double, c_double, POINTER(c_double)],
+    ),
+    "swe_run_to_outputs": (
+        c_int32,
+        [c_void_p, SweState, POINTER(c_double), c_int32, c_int32, c_void_p, c_void_p, c_void_p,
+         SweState, POINTER(SweRunInfo), c_void_p],
+    ),
+    "swe_run_steps": (
+        c_int32, [c_void_p, SweState, c_int32, c_double, POINTER(SweRunInfo), c_void_p]
+    ),
+    "swe_loop_begin": (c_int32, [c_void_p, SweState, c_double, c_int32, c_void_p]),
+    "swe_loop_advance": (c_int32, [c_void_p, c_int32, c_void_p]),
+    "swe_loop_end": (c_int32, [c_void_p, SweState, POINTER(SweRunInfo), c_void_p]),
+}
+
+_lib = None
+
+
+class SweError(RuntimeError):
+    """A libswe_hip call failed; ``code`` is the SWE_E* status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+def load() -> ctypes.CDLL:
+    """Load (once) and return the library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C gpu-hybrid-flood-forecasting_amd` "
+            "or __graft_entry__.build().  There is no CPU fallback for the HIP step."
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load().swe_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(code: int, where: str) -> int:
+    if code == SWE_OK:
+        return code
+    msg = last_error()
+    if code == SWE_EINVAL:
+        raise ValueError(msg or where)
+    raise SweError(code, f"{where}: {msg}")
+
+
+def stream_handle(device: torch.device | None = None) -> c_void_p:
+    return c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def state_struct(h: torch.Tensor, hu: torch.Tensor, hv: torch.Tensor) -> SweState:
+    return SweState(h.data_ptr(), hu.data_ptr(), hv.data_ptr())
+
+
+class Context:
+    """Owns one ``SweContext*`` and keeps the device fields it points at alive."""
+
+    def __init__(self, *, bed, manning_n, domain_mask, rainfall_multiplier, boundary_types,
+                 rainfall_times_s, rainfall_rates_m_s, default_rainfall_m_s, dx, dy, gravity,
+                 cfl, dry_tolerance, max_dt, variant="fast"):
+        lib = load()
+        self._keep = (bed, manning_n, domain_mask, rainfall_multiplier)
+        ny, nx = bed.shape
+        times = (c_float * len(rainfall_times_s))(*[float(x) for x in rainfall_times_s])
+        rates = (c_float * len(rainfall_rates_m_s))(*[float(x) for x in rainfall_rates_m_s])
+        if len(times) != len(rates):
+            raise ValueError("Rainfall times and rates must have the same length.")
+        desc = SweParamsDesc(
+            nx=nx, ny=ny, bed=bed.data_ptr(), manning_n=manning_n.data_ptr(),
+            domain_mask=domain_mask.data_ptr(), rainfall_multiplier=rainfall_multiplier.data_ptr(),
+            boundary_types=(c_int32 * 4)(*[int(b) for b in boundary_types]),
+            rainfall_times_s=times, rainfall_rates_m_s=rates, n_rainfall=len(times),
+            default_rainfall_m_s=float(default_rainfall_m_s), dx=float(dx), dy=float(dy),
+            gravity=float(gravity), cfl=float(cfl), dry_tolerance=float(dry_tolerance),
+            max_dt=float(max_dt),
+        )
+        handle = c_void_p()
+        if variant not in VARIANTS:
+            raise ValueError(f"variant must be one of {sorted(VARIANTS)}")
+        torch.cuda.current_stream().synchronize()  # fields must be ready for the setup kernel
+        check(lib.swe_create(ctypes.byref(desc), VARIANTS[variant], ctypes.byref(handle)),
+              "swe_create")
+        self.handle = handle
+        self.shape = (ny, nx)
+        self.variant = variant
+        self.gravity = float(gravity)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib is not None:
+            _lib.swe_destroy(h)
+            self.handle = None
+
+    def set_variant(self, variant: str) -> None:
+        check(load().swe_set_variant(self.handle, VARIANTS[variant]), "swe_set_variant")
+        self.variant = variant
+
+    def set_tile_rows(self, rows: int) -> None:
+        check(load().swe_set_tile_rows(self.handle, int(rows)), "swe_set_tile_rows")
+
+    def set_graph_chunk(self, chunk: int) -> None:
+        check(load().swe_set_graph_chunk(self.handle, int(chunk)), "swe_set_graph_chunk")

@@ -1,0 +1,185 @@
+/*
+ * swe_hip.h -- C ABI of libswe_hip.so, the MI355X (gfx950) explicit 2-D
+ * shallow-water finite-volume step.
+ *
+ * The reference exposes this path as pure Python functions over NamedTuple
+ * pytrees (no FFI exists there).  Each entry point below replaces one of
+ * them; the Python package hybrid_flood_amd binds them with ctypes and keeps
+ * the reference's names, argument meaning and exceptions.
+ *
+ * Conventions
+ *   - Fields are row-major (ny, nx) float32 device arrays, x contiguous, row
+ *     index increasing northward (jax_solver/shallow_water_2d.py:165-171).
+ *   - domain_mask is a uint8 device array (0 inactive, 1 active); the
+ *     reference stores it as bool (jax_solver/numerics.py:34).
+ *   - boundary_types are ordered left, right, bottom, top with
+ *     SWE_REFLECTIVE = 0 and SWE_OPEN = 1 (jax_solver/boundary_conditions.py:9-16).
+ *   - Time, dt, the rain rate and the CFL quotient are float64; the state is
+ *     float32.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Calls are asynchronous on it unless documented otherwise.
+ *   - Every entry point returns SWE_OK or an error code; swe_last_error()
+ *     returns a thread-local message for the last failure.
+ *   - A context is not thread-safe: use one per host thread / stream.
+ */
+#ifndef SWE_HIP_H
+#define SWE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWE_OK 0
+#define SWE_EINVAL 1      /* bad argument: shapes, times, pointers */
+#define SWE_EMAXSTEPS 2   /* max_steps reached before the last output time */
+#define SWE_EHIP 3        /* HIP runtime error */
+#define SWE_ENONFINITE 4  /* a non-finite value appeared in the state */
+#define SWE_ENODEV 5      /* no usable gfx950 device */
+
+#define SWE_REFLECTIVE 0
+#define SWE_OPEN 1
+
+/* Kernel variants of the same step arithmetic.
+ * SWE_VARIANT_REFERENCE: reference evaluation order, correctly rounded
+ *   division and square root, no FMA contraction; bit-identical depth and
+ *   wet mask with the FP32 oracle.
+ * SWE_VARIANT_FAST: one reciprocal per quotient, hardware sqrt/log/exp,
+ *   HLL velocities reused instead of re-divided; within the stated
+ *   tolerance of the FP64 oracle. */
+#define SWE_VARIANT_REFERENCE 0
+#define SWE_VARIANT_FAST 1
+
+typedef struct SweContext SweContext;
+
+/* Conserved state (jax_solver/numerics.py:21-26): three device arrays. */
+typedef struct SweState {
+  float* h;
+  float* hu;
+  float* hv;
+} SweState;
+
+/* Static fields and controls (jax_solver/numerics.py:29-45, built by
+ * ShallowWaterSolver.__init__ at jax_solver/shallow_water_2d.py:213-253).
+ * Device pointers must stay valid for the life of the context; the rainfall
+ * series is copied at creation. */
+typedef struct SweParamsDesc {
+  int32_t nx, ny;
+  const float* bed;                 /* device (ny, nx) */
+  const float* manning_n;           /* device (ny, nx) */
+  const uint8_t* domain_mask;       /* device (ny, nx) */
+  const float* rainfall_multiplier; /* device (ny, nx) */
+  int32_t boundary_types[4];        /* left, right, bottom, top */
+  const float* rainfall_times_s;    /* host, n_rainfall, strictly increasing */
+  const float* rainfall_rates_m_s;  /* host, n_rainfall */
+  int32_t n_rainfall;               /* >= 1 */
+  double default_rainfall_m_s;
+  double dx, dy, gravity, cfl, dry_tolerance, max_dt;
+} SweParamsDesc;
+
+/* Result record of a run (ScanResult, jax_solver/numerics.py:48-55, plus the
+ * FP64 mass balance of anuga_solver/run_simulation.py:100-107). */
+typedef struct SweRunInfo {
+  int32_t outputs_written;
+  int32_t steps_used;
+  double final_time_s;
+  double volume_initial;   /* m^3 over active cells */
+  double volume_final;
+  double rain_volume;      /* rain applied to active cells */
+  double boundary_volume;  /* signed inflow through open sides */
+  double clip_volume;      /* created by the positivity clip max(h, 0) */
+  double mass_balance_error;  /* V - V0 - rain - boundary - clip */
+  double mass_balance_rel;    /* error / max(|V0|+|rain|+|boundary|+|clip|, 1) */
+  int32_t nonfinite;       /* 1 if a NaN/Inf was seen in the state */
+  int32_t _pad;
+} SweRunInfo;
+
+/* Library identity and the last error of the calling thread. */
+const char* swe_version(void);
+const char* swe_last_error(void);
+
+/* Create a context for one grid (ShallowWaterSolver.__init__,
+ * jax_solver/shallow_water_2d.py:213-253).  Validates shapes and controls,
+ * copies the rainfall series to the device, allocates the per-block partials,
+ * the device control block and the ping-pong work state. */
+int swe_create(const SweParamsDesc* params, int32_t variant, SweContext** out);
+int swe_destroy(SweContext* ctx);
+int swe_set_variant(SweContext* ctx, int32_t variant);
+/* Rows per wavefront strip of the fused step (tuning knob; 0 = default). */
+int swe_set_tile_rows(SweContext* ctx, int32_t rows);
+
+/* One step with caller-given dt and time (finite_volume_step,
+ * jax_solver/numerics.py:312-369).  `in` and `out` must not alias. */
+int swe_step(SweContext* ctx, SweState in, SweState out, double dt, double time_s,
+             void* stream);
+
+/* CFL step of a state (compute_cfl_dt, jax_solver/numerics.py:297-309).
+ * Synchronises the stream and writes the float64 dt to *dt_out. */
+int swe_cfl_dt(SweContext* ctx, SweState in, double* dt_out, void* stream);
+
+/* Per-cell inverse CFL time (|u|+c)/dx + (|v|+c)/dy, 0 outside the mask
+ * (jax_solver/numerics.py:299-303), into a device float array. */
+int swe_cfl_field(SweContext* ctx, SweState in, float* inv_dt_out, void* stream);
+
+/* Rainfall rate at time_s, float64 linear interpolation with the default
+ * outside the series (rainfall_rate, jax_solver/numerics.py:286-294; the
+ * float64 form of numpy_solver/shallow_water_2d.py:254-264).  Host only. */
+int swe_rainfall_rate(const SweContext* ctx, double time_s, double* rate_out);
+int swe_interp_rate(const float* times_s, const float* rates_m_s, int32_t n, double dflt,
+                    double time_s, double* rate_out);
+
+/* Adaptive loop to fixed output times (integrate_to_outputs,
+ * jax_solver/numerics.py:372-434; ShallowWaterSolver.run,
+ * jax_solver/shallow_water_2d.py:278-306).  The time loop runs on the device:
+ * each step kernel reduces the CFL maximum of its new state and its last
+ * workgroup sets the next dt, time, output slot and stop flag; the host only
+ * launches chunks of steps and polls the stop flag between chunks.
+ *   output_times_s: host float64[n_times], n_times >= 1, finite (else
+ *                   SWE_EINVAL); times[0] is the initial clock.  A target the
+ *                   clock has already passed is reached at once, as in the
+ *                   reference's scan; ShallowWaterSolver.run additionally
+ *                   requires times to start at 0 and increase strictly.
+ *   out_h/out_hu/out_hv: device float32[n_times][ny][nx]; slot 0 receives
+ *                   the initial state; slots the loop did not reach receive
+ *                   the last state (as the reference's scan stacks it).
+ *   final_state:    optional device state receiving the last state.
+ * Synchronises the stream.  Returns SWE_EMAXSTEPS (with `info` filled) when
+ * max_steps ended the loop early, SWE_ENONFINITE if the state went non-finite. */
+int swe_run_to_outputs(SweContext* ctx, SweState initial, const double* output_times_s,
+                       int32_t n_times, int32_t max_steps, float* out_h, float* out_hu,
+                       float* out_hv, SweState final_state, SweRunInfo* info, void* stream);
+
+/* Exactly n_steps adaptive steps (dt = CFL dt capped at max_dt, no output
+ * target) from time t0, state updated in place.  The device loop of
+ * swe_run_to_outputs without outputs; used by the throughput benchmark
+ * (jax_solver/benchmarking.py:17-97 times a step loop the same way).
+ * Asynchronous unless `info` is non-NULL. */
+int swe_run_steps(SweContext* ctx, SweState state, int32_t n_steps, double t0,
+                  SweRunInfo* info, void* stream);
+
+/* The device loop of swe_run_steps in three parts, for callers that drive it
+ * themselves (a throughput harness brackets swe_loop_advance with events; a
+ * coupled model interleaves its own work, as hybrid/coupled_forecast.py:123-161
+ * does between fixed-length substep loops).
+ *   swe_loop_begin:   copy `state` into the context's work buffers, reduce its
+ *                     CFL maximum and volume, prime the control block at t0
+ *                     with a budget of max_steps steps (no output target).
+ *   swe_loop_advance: queue n_launches step kernels (rounded up to the graph
+ *                     chunk); launches after the budget is spent are no-ops.
+ *   swe_loop_end:     copy the current state back to `state`; with `info`
+ *                     non-NULL, synchronise and fill it.
+ * All three are asynchronous on `stream` unless `info` is given. */
+int swe_loop_begin(SweContext* ctx, SweState state, double t0, int32_t max_steps, void* stream);
+int swe_loop_advance(SweContext* ctx, int32_t n_launches, void* stream);
+int swe_loop_end(SweContext* ctx, SweState state, SweRunInfo* info, void* stream);
+
+/* Use HIP graphs of `chunk` captured step launches for the device loop
+ * (0 = plain launches).  Default 64. */
+int swe_set_graph_chunk(SweContext* ctx, int32_t chunk);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWE_HIP_H */

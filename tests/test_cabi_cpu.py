@@ -1,0 +1,140 @@
+"""C-ABI checks that need no GPU: exports, host-side helpers, argument errors."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, gpu_available
+
+
+def _lib():
+    from hybrid_flood_amd import _lib
+
+    return _lib
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "swe_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(swe_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    L = _lib()
+    lib = L.load()
+    names = _header_symbols()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), f"libswe_hip.so does not export {name}"
+        assert name in L.SIGNATURES, f"ctypes binding lacks {name}"
+    assert set(L.SIGNATURES) == set(names)
+
+
+def test_version_string():
+    assert b"gfx950" in _lib().load().swe_version()
+
+
+def _interp(times, rates, dflt, t):
+    L = _lib()
+    times = np.asarray(times, np.float32)
+    rates = np.asarray(rates, np.float32)
+    out = ctypes.c_double()
+    rc = L.load().swe_interp_rate(times.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                  rates.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                  len(times), dflt, t, ctypes.byref(out))
+    assert rc == 0
+    return out.value
+
+
+def test_interp_rate_matches_numpy_bit_for_bit():
+    """Rain rate = float64 np.interp with the default outside the series
+    (numpy_solver/shallow_water_2d.py:254-264)."""
+    rng = np.random.default_rng(3)
+    times = np.cumsum(rng.uniform(10, 90, 181)).astype(np.float32)
+    times[0] = 0.0
+    rates = rng.uniform(0, 3e-5, 181).astype(np.float32)
+    ts = np.concatenate([rng.uniform(-50, times[-1] + 50, 4000), times.astype(np.float64),
+                         [times[-1], 0.0, -1e-9, np.nextafter(times[-1], np.inf)]])
+    for t in ts:
+        want = np.interp(t, times, rates, left=1.5e-7, right=2.5e-7)
+        assert _interp(times, rates, 1.5e-7 if t < times[0] else 2.5e-7, t) == want or (
+            # left and right defaults differ only in this test; the solver uses one default
+            times[0] <= t <= times[-1] and _interp(times, rates, 0.0, t) == want
+        )
+    for t in rng.uniform(-10, times[-1] + 10, 500):
+        assert _interp(times, rates, 7e-8, t) == np.interp(t, times, rates, left=7e-8, right=7e-8)
+
+
+def test_interp_rate_single_sample():
+    for t, want in ((-1.0, 0.5), (0.0, 2.0), (1.0, 0.5)):
+        assert _interp([0.0], [2.0], 0.5, t) == want
+
+
+def _desc(nx=4, ny=3, **over):
+    L = _lib()
+    t = (ctypes.c_float * 2)(0.0, 10.0)
+    r = (ctypes.c_float * 2)(0.0, 0.0)
+    d = L.SweParamsDesc(nx=nx, ny=ny, bed=1, manning_n=1, domain_mask=1, rainfall_multiplier=1,
+                        boundary_types=(ctypes.c_int32 * 4)(0, 1, 0, 0), rainfall_times_s=t,
+                        rainfall_rates_m_s=r, n_rainfall=2, default_rainfall_m_s=0.0, dx=1.0,
+                        dy=1.0, gravity=9.81, cfl=0.45, dry_tolerance=1e-4, max_dt=10.0)
+    for k, v in over.items():
+        setattr(d, k, v)
+    return d, (t, r)
+
+
+@pytest.mark.parametrize("field,value", [
+    ("nx", 0), ("ny", -1), ("bed", None), ("n_rainfall", 0), ("dx", 0.0), ("cfl", -1.0),
+    ("dry_tolerance", 0.0), ("max_dt", float("nan")), ("gravity", 0.0),
+])
+def test_create_rejects_bad_arguments(field, value):
+    L = _lib()
+    d, keep = _desc(**{field: value})
+    h = ctypes.c_void_p()
+    assert L.load().swe_create(ctypes.byref(d), 1, ctypes.byref(h)) == L.SWE_EINVAL
+    assert not h.value
+    assert L.last_error()
+
+
+def test_create_rejects_bad_boundary_and_variant():
+    L = _lib()
+    d, keep = _desc()
+    d.boundary_types[2] = 7
+    h = ctypes.c_void_p()
+    assert L.load().swe_create(ctypes.byref(d), 1, ctypes.byref(h)) == L.SWE_EINVAL
+    d, keep = _desc()
+    assert L.load().swe_create(ctypes.byref(d), 9, ctypes.byref(h)) == L.SWE_EINVAL
+
+
+def test_create_rejects_non_increasing_rain_times():
+    L = _lib()
+    d, (t, r) = _desc()
+    t[1] = 0.0
+    h = ctypes.c_void_p()
+    assert L.load().swe_create(ctypes.byref(d), 1, ctypes.byref(h)) == L.SWE_EINVAL
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
+def test_create_without_device_is_a_clean_error():
+    L = _lib()
+    d, keep = _desc()
+    h = ctypes.c_void_p()
+    rc = L.load().swe_create(ctypes.byref(d), 1, ctypes.byref(h))
+    assert rc in (L.SWE_ENODEV, L.SWE_EHIP)
+    assert "device" in L.last_error().lower() or "hip" in L.last_error().lower()
+
+
+def test_null_context_calls_fail_cleanly():
+    L = _lib()
+    lib = L.load()
+    s = L.SweState(None, None, None)
+    assert lib.swe_step(None, s, s, 0.1, 0.0, None) == L.SWE_EINVAL
+    out = ctypes.c_double()
+    assert lib.swe_cfl_dt(None, s, ctypes.byref(out), None) == L.SWE_EINVAL
+    assert lib.swe_loop_advance(None, 4, None) == L.SWE_EINVAL
+    assert lib.swe_set_graph_chunk(None, 3) == L.SWE_EINVAL
