@@ -129,22 +129,27 @@ struct FaceFlux {
 // tangential velocities of the original depth of each side.  XDIR selects the
 // x-face tangential flux q_n * v_t' (numerics.py:84) instead of the y-face
 // q_t * v_n' (numerics.py:120).  Follows numerics.py:162-224 / 227-283.
-template <class M, bool XDIR>
+// MASKED = false is the same arithmetic for a face whose two cells are both
+// active (no wall mirroring, always relevant); callers use it when a whole
+// wavefront's faces are interior to the domain.
+template <class M, bool XDIR, bool MASKED = true>
 __device__ __forceinline__ FaceFlux face_flux(const Grid& G, float hA, float unA, float utA,
                                               float bA, int aA, float hB, float unB,
                                               float utB, float bB, int aB) {
-  // An active/inactive face is a wall: the inactive side becomes the mirror
-  // image of the active one (numerics.py:183-192).
-  const bool wallA = aA && !aB;
-  const bool wallB = !aA && aB;
-  hB = wallA ? hA : hB;
-  unB = wallA ? -unA : unB;
-  utB = wallA ? utA : utB;
-  bB = wallA ? bA : bB;
-  hA = wallB ? hB : hA;
-  unA = wallB ? -unB : unA;
-  utA = wallB ? utB : utA;
-  bA = wallB ? bB : bA;
+  if (MASKED) {
+    // An active/inactive face is a wall: the inactive side becomes the mirror
+    // image of the active one (numerics.py:183-192).
+    const bool wallA = aA && !aB;
+    const bool wallB = !aA && aB;
+    hB = wallA ? hA : hB;
+    unB = wallA ? -unA : unB;
+    utB = wallA ? utA : utB;
+    bB = wallA ? bA : bB;
+    hA = wallB ? hB : hA;
+    unA = wallB ? -unB : unA;
+    utA = wallB ? utB : utA;
+    bA = wallB ? bB : bA;
+  }
 
   const float dry = G.dry;
   const float bf = fmaxf(bA, bB);
@@ -180,7 +185,7 @@ __device__ __forceinline__ FaceFlux face_flux(const Grid& G, float hA, float unA
   const float mt = M::quot((sB * FtA - sA * FtB) + sAB * (QtB - QtA), den, rden);
   const bool upA = sA >= 0.0f;
   const bool upB = sB <= 0.0f;
-  const bool rel = aA || aB;
+  const bool rel = !MASKED || aA || aB;
   FaceFlux F;
   F.f0 = rel ? (upA ? QnA : (upB ? QnB : m0)) : 0.0f;
   F.fn = rel ? (upA ? FnA : (upB ? FnB : mn)) : 0.0f;

@@ -1,0 +1,569 @@
+// Batched ensemble of independent shallow-water domains of uneven size.
+//
+// Every launch of k_ensemble_step advances each unfinished member by one of
+// its own adaptive steps.  The launch is persistent: a fixed grid sized to
+// the resident capacity of the device walks the flattened list of all
+// members' strips (62 columns x `rows` rows each) with a grid stride, so
+// members of 256^2 and 2048^2 cells share one launch and one load balance.
+// Each member keeps its own control block (dt, clock, step count) and its
+// own per-wavefront partials; the wavefront that finishes a member's last
+// strip reduces that member's CFL maximum and FP64 mass terms and plans the
+// member's next step, so no member waits for another.  The host captures
+// chunks of launches in a HIP graph and polls the members' stop flags
+// between chunks.
+//
+// The arithmetic per member is exactly the single-domain step
+// (swe_strip.hpp / swe_device.hpp), i.e. jax_solver/numerics.py:312-369
+// with the device loop of numerics.py:372-434.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/swe_hip.h"
+#include "swe_device.hpp"
+#include "swe_loop.hpp"
+#include "swe_strip.hpp"
+#include "swe_host.hpp"
+
+namespace swe {
+
+struct Member {
+  Grid G;
+  LoopConst L;
+  LoopControl* ctl;
+  float* buf[2];  // two contiguous [h | hu | hv] work states
+  int segs_x, rows, n_waves, wave_base;
+};
+
+struct EnsPartials {
+  float* inv;
+  double* vol;
+  double* clip;
+  double* bnd;
+  int* nf;
+};
+
+__device__ __forceinline__ int wave_max_int(int x) {
+  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, kWave));
+  return x;
+}
+
+// Member m with wave_base[m] <= w < wave_base[m+1] (wave-uniform).
+__device__ __forceinline__ int find_member(const Member* __restrict__ mem, int n, int w) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mem[mid].wave_base <= w)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class M>
+__global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict__ mem,
+                                                      int n_members, int total_waves,
+                                                      EnsPartials P) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int stride = gridDim.x * (blockDim.x / kWave);
+  for (int w = blockIdx.x * (blockDim.x / kWave) + (int)(threadIdx.x / kWave); w < total_waves;
+       w += stride) {
+    const int m = find_member(mem, n_members, w);
+    const Member& mb = mem[m];
+    LoopControl* ctl = mb.ctl;
+    if (ctl->done) continue;
+    const int local = w - mb.wave_base;
+    const size_t N = (size_t)mb.G.nx * (size_t)mb.G.ny;
+    const int par = ctl->steps & 1;
+    const float* src = mb.buf[par];
+    float* dst = mb.buf[par ^ 1];
+    StripAcc acc;
+    march_strip<M, true>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
+                         local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dt_f, ctl->dtrate_f,
+                         ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr, nullptr, nullptr, acc);
+    const float inv = wave_max(acc.inv);
+    const double vol = wave_sum(acc.vol);
+    const double clip = wave_sum(acc.clip);
+    const double bnd = wave_sum(acc.bnd);
+    const int nf = wave_or(acc.nf);
+    int last = 0;
+    if (lane == 0) {
+      __hip_atomic_store(P.inv + w, inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(P.vol + w, vol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(P.clip + w, clip, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(P.bnd + w, bnd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(P.nf + w, nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned prev =
+          __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      last = prev == (unsigned)(mb.n_waves - 1);
+    }
+    last = __shfl(last, 0, kWave);
+    if (!last) continue;
+    // This wavefront closes member m's step.
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    float ginv = 0.0f;
+    double gvol = 0.0, gclip = 0.0, gbnd = 0.0;
+    int gnf = 0;
+    for (int k = lane; k < mb.n_waves; k += kWave) {
+      const int q = mb.wave_base + k;
+      ginv = fmaxf(ginv, coherent_load(P.inv + q));
+      gvol += coherent_load(P.vol + q);
+      gclip += coherent_load(P.clip + q);
+      gbnd += coherent_load(P.bnd + q);
+      gnf |= coherent_load(P.nf + q);
+    }
+    Totals tot;
+    tot.inv = wave_max(ginv);
+    tot.vol = wave_sum(gvol);
+    tot.clip = wave_sum(gclip);
+    tot.bnd = wave_sum(gbnd);
+    tot.nf = wave_or(gnf);
+    const double t_new = ctl->t + ctl->dt;
+    int rj = 0;
+    for (int k = lane; k < mb.L.n_rain - 1; k += kWave)
+      if (mb.L.rain_t[k] <= t_new && t_new < mb.L.rain_t[k + 1]) rj = k;
+    rj = wave_max_int(rj);
+    if (lane == 0) finish_step(ctl, mb.L, tot, t_new, rj);
+  }
+}
+
+}  // namespace swe
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+using namespace swe;
+
+namespace {
+int efail(int code, const std::string& m) { return swe::set_error(code, m); }
+#define ENS_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return efail(SWE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+}  // namespace
+
+struct SweEnsemble {
+  int variant = SWE_VARIANT_FAST;
+  int rows = 16;
+  int graph_chunk = 32;
+  std::vector<SweMemberDesc> desc;
+  std::vector<Member> host;
+  std::vector<std::vector<double>> rain;  // per member [t | r]
+  Member* d_mem = nullptr;
+  LoopControl* d_ctl = nullptr;
+  LoopControl* h_ctl = nullptr;  // pinned, n members
+  double* d_rain = nullptr;      // all members' series
+  double* d_times = nullptr;     // per member {0, t_end}
+  float* d_work = nullptr;       // all members' A|B states
+  EnsPartials P{};
+  int parts_cap = 0;
+  int total_waves = 0;
+  int grid = 0;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  double times_host[2] = {0.0, 0.0};
+  bool active = false;
+  std::vector<int> steps_at_begin;
+};
+
+namespace {
+
+int layout(SweEnsemble* e) {
+  int base = 0;
+  for (auto& m : e->host) {
+    m.rows = e->rows;
+    m.segs_x = (m.G.nx + kStripCols - 1) / kStripCols;
+    m.n_waves = m.segs_x * ((m.G.ny + m.rows - 1) / m.rows);
+    m.wave_base = base;
+    base += m.n_waves;
+  }
+  e->total_waves = base;
+  if (base > e->parts_cap) {
+    if (e->P.vol) ENS_TRY(hipFree(e->P.vol));
+    const size_t bytes = (size_t)base * (3 * sizeof(double) + sizeof(float) + sizeof(int));
+    char* p = nullptr;
+    ENS_TRY(hipMalloc(&p, bytes));
+    e->P.vol = reinterpret_cast<double*>(p);
+    e->P.clip = e->P.vol + base;
+    e->P.bnd = e->P.clip + base;
+    e->P.inv = reinterpret_cast<float*>(e->P.bnd + base);
+    e->P.nf = reinterpret_cast<int*>(e->P.inv + base);
+    e->parts_cap = base;
+  }
+  int dev = 0, cus = 0, per_cu = 0;
+  ENS_TRY(hipGetDevice(&dev));
+  ENS_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (e->variant == SWE_VARIANT_REFERENCE)
+    ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<RefMath>, 256, 0));
+  else
+    ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<FastMath>, 256, 0));
+  e->grid = std::max(1, std::min(cus * std::max(per_cu, 1), (base + 3) / 4));
+  if (e->graph) {
+    (void)hipGraphExecDestroy(e->graph);
+    e->graph = nullptr;
+  }
+  return SWE_OK;
+}
+
+void launch_ens(SweEnsemble* e, hipStream_t s) {
+  if (e->variant == SWE_VARIANT_REFERENCE)
+    hipLaunchKernelGGL(k_ensemble_step<RefMath>, dim3(e->grid), dim3(256), 0, s, e->d_mem,
+                       (int)e->host.size(), e->total_waves, e->P);
+  else
+    hipLaunchKernelGGL(k_ensemble_step<FastMath>, dim3(e->grid), dim3(256), 0, s, e->d_mem,
+                       (int)e->host.size(), e->total_waves, e->P);
+}
+
+int launch_chunks(SweEnsemble* e, int launches, hipStream_t s) {
+  if (e->graph_chunk <= 0) {
+    for (int k = 0; k < launches; ++k) launch_ens(e, s);
+    ENS_TRY(hipGetLastError());
+    return SWE_OK;
+  }
+  if (!e->graph) {
+    if (!e->cap_stream) ENS_TRY(hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking));
+    ENS_TRY(hipStreamBeginCapture(e->cap_stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < e->graph_chunk; ++k) launch_ens(e, e->cap_stream);
+    hipGraph_t g = nullptr;
+    ENS_TRY(hipStreamEndCapture(e->cap_stream, &g));
+    hipError_t err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (err != hipSuccess) return efail(SWE_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+  }
+  for (int k = 0; k < launches; k += e->graph_chunk) ENS_TRY(hipGraphLaunch(e->graph, s));
+  return SWE_OK;
+}
+
+// Launch per-member init reductions (CFL, volume) that prime each member's control.
+template <class MM>
+__global__ __launch_bounds__(256) void k_ens_init(const Member* __restrict__ mem, int m,
+                                                 EnsPartials P) {
+  // One workgroup per member: a grid-stride pass over the member's cells.
+  const Member& mb = mem[m];
+  const Grid& G = mb.G;
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  const float* H = mb.buf[0];
+  float inv = 0.0f;
+  double vol = 0.0;
+  int nf = 0;
+  for (size_t k = threadIdx.x; k < N; k += blockDim.x) {
+    const float h = H[k], hu = H[N + k], hv = H[2 * N + k];
+    const int a = G.mask[k] != 0;
+    inv = fmaxf(inv, cfl_inverse<MM>(G, h, hu, hv, a));
+    vol += a ? (double)h : 0.0;
+    nf |= !(isfinite(h) && isfinite(hu) && isfinite(hv));
+  }
+  const Totals tot = block_reduce(inv, vol, 0.0, 0.0, nf);
+  const double t0 = mb.L.times[0];
+  __shared__ int s_j;
+  if (threadIdx.x == 0) s_j = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < mb.L.n_rain - 1; k += blockDim.x)
+    if (mb.L.rain_t[k] <= t0 && t0 < mb.L.rain_t[k + 1]) s_j = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    LoopControl* ctl = mb.ctl;
+    ctl->t = t0;
+    ctl->dt = 0.0;
+    ctl->steps = 0;
+    ctl->vol0 = ctl->vol = tot.vol * mb.L.cell_area;
+    ctl->rain = ctl->bnd = ctl->clip = 0.0;
+    ctl->nonfinite = tot.nf;
+    ctl->max_inv = (double)tot.inv;
+    ctl->counter = 0;
+    ctl->out_idx = 1;
+    if (mb.L.n_times < 2 || mb.L.max_steps <= 0 || tot.nf || t0 >= mb.L.times[1] - kOutputSlack) {
+      ctl->done = 1;
+      ctl->rec_lo = ctl->rec_hi = 0;
+    } else {
+      ctl->done = 0;
+      plan_next_step(ctl, mb.L, tot.inv, t0, 1, s_j);
+    }
+  }
+  (void)P;
+}
+
+__global__ void k_ens_copy(const float* __restrict__ src, float* __restrict__ h,
+                           float* __restrict__ hu, float* __restrict__ hv, size_t N, int to_dst) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * blockDim.x) {
+    if (to_dst) {
+      h[k] = src[k];
+      hu[k] = src[N + k];
+      hv[k] = src[2 * N + k];
+    } else {
+      float* d = const_cast<float*>(src);
+      d[k] = h[k];
+      d[N + k] = hu[k];
+      d[2 * N + k] = hv[k];
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant, SweEnsemble** out) {
+  if (!members || !out || n < 1) return efail(SWE_EINVAL, "swe_ensemble_create: need >= 1 member");
+  *out = nullptr;
+  if (variant != SWE_VARIANT_REFERENCE && variant != SWE_VARIANT_FAST)
+    return efail(SWE_EINVAL, "swe_ensemble_create: unknown kernel variant");
+  for (int m = 0; m < n; ++m) {
+    const SweParamsDesc& p = members[m].params;
+    if (p.nx < 1 || p.ny < 1 || !p.bed || !p.manning_n || !p.domain_mask || !p.rainfall_multiplier ||
+        p.n_rainfall < 1 || !p.rainfall_times_s || !p.rainfall_rates_m_s || !(p.dx > 0) ||
+        !(p.dy > 0) || !(p.gravity > 0) || !(p.cfl > 0) || !(p.dry_tolerance > 0) || !(p.max_dt > 0))
+      return efail(SWE_EINVAL, "swe_ensemble_create: member " + std::to_string(m) + " has invalid parameters");
+    for (int k = 0; k < 4; ++k)
+      if (p.boundary_types[k] != SWE_REFLECTIVE && p.boundary_types[k] != SWE_OPEN)
+        return efail(SWE_EINVAL, "swe_ensemble_create: bad boundary type");
+    for (int k = 1; k < p.n_rainfall; ++k)
+      if (!(p.rainfall_times_s[k] > p.rainfall_times_s[k - 1]))
+        return efail(SWE_EINVAL, "swe_ensemble_create: rainfall times must increase strictly");
+    const SweState& s = members[m].state;
+    if (!s.h || !s.hu || !s.hv) return efail(SWE_EINVAL, "swe_ensemble_create: null member state");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return efail(SWE_ENODEV, "swe_ensemble_create: no HIP device is visible");
+  SweEnsemble* e = new SweEnsemble();
+  e->variant = variant;
+  e->desc.assign(members, members + n);
+  e->host.resize(n);
+  auto bail = [&](int rc) {
+    swe_ensemble_destroy(e);
+    return rc;
+  };
+  size_t rain_total = 0, cells_total = 0;
+  for (int m = 0; m < n; ++m) {
+    rain_total += 2 * (size_t)members[m].params.n_rainfall;
+    cells_total += (size_t)members[m].params.nx * members[m].params.ny;
+  }
+  if (hipMalloc(&e->d_rain, sizeof(double) * rain_total) != hipSuccess ||
+      hipMalloc(&e->d_times, sizeof(double) * 2 * n) != hipSuccess ||
+      hipMalloc(&e->d_ctl, sizeof(LoopControl) * n) != hipSuccess ||
+      hipHostMalloc(&e->h_ctl, sizeof(LoopControl) * n, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&e->d_work, sizeof(float) * 6 * cells_total) != hipSuccess ||
+      hipMalloc(&e->d_mem, sizeof(Member) * n) != hipSuccess)
+    return bail(efail(SWE_EHIP, "swe_ensemble_create: device allocation failed"));
+  if (hipMemset(e->d_ctl, 0, sizeof(LoopControl) * n) != hipSuccess)
+    return bail(efail(SWE_EHIP, "swe_ensemble_create: clear control"));
+  std::vector<double> rain_all;
+  rain_all.reserve(rain_total);
+  size_t rain_off = 0, cell_off = 0;
+  for (int m = 0; m < n; ++m) {
+    const SweParamsDesc& p = members[m].params;
+    Member& mb = e->host[m];
+    Grid& G = mb.G;
+    G.nx = p.nx;
+    G.ny = p.ny;
+    for (int k = 0; k < 4; ++k) G.bt[k] = p.boundary_types[k];
+    G.g = (float)p.gravity;
+    G.half_g = (float)(0.5 * p.gravity);
+    G.dry = (float)p.dry_tolerance;
+    G.dx = (float)p.dx;
+    G.dy = (float)p.dy;
+    G.rdx = 1.0f / G.dx;
+    G.rdy = 1.0f / G.dy;
+    G.bed = p.bed;
+    G.n = p.manning_n;
+    G.mask = p.domain_mask;
+    G.mult = p.rainfall_multiplier;
+    LoopConst& L = mb.L;
+    L.cfl = p.cfl;
+    L.max_dt = p.max_dt;
+    L.gravity = p.gravity;
+    L.cell_area = p.dx * p.dy;
+    L.rain_default = p.default_rainfall_m_s;
+    L.dxd = p.dx;
+    L.dyd = p.dy;
+    L.rain_t = e->d_rain + rain_off;
+    L.rain_r = e->d_rain + rain_off + p.n_rainfall;
+    L.times = e->d_times + 2 * m;
+    L.n_rain = p.n_rainfall;
+    L.n_times = 2;
+    for (int k = 0; k < p.n_rainfall; ++k) rain_all.push_back(p.rainfall_times_s[k]);
+    for (int k = 0; k < p.n_rainfall; ++k) rain_all.push_back(p.rainfall_rates_m_s[k]);
+    rain_off += 2 * (size_t)p.n_rainfall;
+    mb.ctl = e->d_ctl + m;
+    const size_t N = (size_t)p.nx * p.ny;
+    mb.buf[0] = e->d_work + cell_off;
+    mb.buf[1] = e->d_work + cell_off + 3 * N;
+    cell_off += 6 * N;
+  }
+  if (hipMemcpy(e->d_rain, rain_all.data(), sizeof(double) * rain_all.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(efail(SWE_EHIP, "swe_ensemble_create: upload rain"));
+  // FP64 rain-multiplier sums over active cells, one member at a time (setup only).
+  for (int m = 0; m < n; ++m) {
+    const SweParamsDesc& p = members[m].params;
+    const size_t N = (size_t)p.nx * p.ny;
+    std::vector<float> mult(N);
+    std::vector<uint8_t> mask(N);
+    if (hipMemcpy(mult.data(), p.rainfall_multiplier, 4 * N, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(mask.data(), p.domain_mask, N, hipMemcpyDeviceToHost) != hipSuccess)
+      return bail(efail(SWE_EHIP, "swe_ensemble_create: read multiplier"));
+    double s = 0.0;
+    for (size_t k = 0; k < N; ++k)
+      if (mask[k]) s += (double)mult[k];
+    e->host[m].L.sum_mult = s;
+  }
+  int rc = layout(e);
+  if (rc) return bail(rc);
+  *out = e;
+  return SWE_OK;
+}
+
+int swe_ensemble_destroy(SweEnsemble* e) {
+  if (!e) return SWE_OK;
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  if (e->cap_stream) (void)hipStreamDestroy(e->cap_stream);
+  if (e->P.vol) (void)hipFree(e->P.vol);
+  if (e->d_mem) (void)hipFree(e->d_mem);
+  if (e->d_work) (void)hipFree(e->d_work);
+  if (e->h_ctl) (void)hipHostFree(e->h_ctl);
+  if (e->d_ctl) (void)hipFree(e->d_ctl);
+  if (e->d_times) (void)hipFree(e->d_times);
+  if (e->d_rain) (void)hipFree(e->d_rain);
+  delete e;
+  return SWE_OK;
+}
+
+int swe_ensemble_set_tile_rows(SweEnsemble* e, int32_t rows) {
+  if (!e || rows < 1 || rows > 4096) return efail(SWE_EINVAL, "swe_ensemble_set_tile_rows: rows in [1, 4096]");
+  if (e->active) return efail(SWE_EINVAL, "swe_ensemble_set_tile_rows: not while a run is active");
+  e->rows = rows;
+  return layout(e);
+}
+
+int swe_ensemble_set_graph_chunk(SweEnsemble* e, int32_t chunk) {
+  if (!e || chunk < 0 || chunk > 4096) return efail(SWE_EINVAL, "swe_ensemble_set_graph_chunk: chunk in [0, 4096]");
+  e->graph_chunk = chunk;
+  if (e->graph) {
+    (void)hipGraphExecDestroy(e->graph);
+    e->graph = nullptr;
+  }
+  return SWE_OK;
+}
+
+int swe_ensemble_begin(SweEnsemble* e, double t_end, int32_t max_steps, void* stream) {
+  if (!e) return efail(SWE_EINVAL, "swe_ensemble_begin: null ensemble");
+  if (!(t_end > 0.0) || !std::isfinite(t_end) || max_steps < 0)
+    return efail(SWE_EINVAL, "swe_ensemble_begin: need t_end > 0 and max_steps >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  const int n = (int)e->host.size();
+  std::vector<double> times(2 * n);
+  for (int m = 0; m < n; ++m) {
+    times[2 * m] = 0.0;
+    times[2 * m + 1] = t_end;
+    e->host[m].L.max_steps = max_steps;
+  }
+  ENS_TRY(hipMemcpy(e->d_times, times.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+  ENS_TRY(hipMemcpy(e->d_mem, e->host.data(), sizeof(Member) * n, hipMemcpyHostToDevice));
+  for (int m = 0; m < n; ++m) {
+    const SweState& st = e->desc[m].state;
+    const size_t N = (size_t)e->host[m].G.nx * e->host[m].G.ny;
+    const int nb = (int)std::min<size_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_ens_copy, dim3(nb), dim3(256), 0, s, e->host[m].buf[0], st.h, st.hu, st.hv, N, 0);
+    if (e->variant == SWE_VARIANT_REFERENCE)
+      hipLaunchKernelGGL(k_ens_init<RefMath>, dim3(1), dim3(256), 0, s, e->d_mem, m, e->P);
+    else
+      hipLaunchKernelGGL(k_ens_init<FastMath>, dim3(1), dim3(256), 0, s, e->d_mem, m, e->P);
+  }
+  ENS_TRY(hipGetLastError());
+  e->steps_at_begin.assign(n, 0);
+  e->active = true;
+  return SWE_OK;
+}
+
+int swe_ensemble_advance(SweEnsemble* e, int32_t n_launches, void* stream) {
+  if (!e || !e->active) return efail(SWE_EINVAL, "swe_ensemble_advance: call swe_ensemble_begin first");
+  if (n_launches < 0) return efail(SWE_EINVAL, "swe_ensemble_advance: n_launches must be >= 0");
+  if (n_launches == 0) return SWE_OK;
+  const int chunk = e->graph_chunk > 0 ? e->graph_chunk : 1;
+  return launch_chunks(e, ((n_launches + chunk - 1) / chunk) * chunk, (hipStream_t)stream);
+}
+
+// Synchronises; returns the number of members still running.
+int swe_ensemble_poll(SweEnsemble* e, int32_t* running, void* stream) {
+  if (!e || !running) return efail(SWE_EINVAL, "swe_ensemble_poll: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  const int n = (int)e->host.size();
+  ENS_TRY(hipMemcpyAsync(e->h_ctl, e->d_ctl, sizeof(LoopControl) * n, hipMemcpyDeviceToHost, s));
+  ENS_TRY(hipStreamSynchronize(s));
+  int r = 0;
+  for (int m = 0; m < n; ++m) r += e->h_ctl[m].done ? 0 : 1;
+  *running = r;
+  return SWE_OK;
+}
+
+int swe_ensemble_end(SweEnsemble* e, SweRunInfo* infos, void* stream) {
+  if (!e || !e->active) return efail(SWE_EINVAL, "swe_ensemble_end: call swe_ensemble_begin first");
+  hipStream_t s = (hipStream_t)stream;
+  const int n = (int)e->host.size();
+  ENS_TRY(hipMemcpyAsync(e->h_ctl, e->d_ctl, sizeof(LoopControl) * n, hipMemcpyDeviceToHost, s));
+  ENS_TRY(hipStreamSynchronize(s));
+  int rc = SWE_OK;
+  for (int m = 0; m < n; ++m) {
+    const LoopControl& k = e->h_ctl[m];
+    const SweState& st = e->desc[m].state;
+    const size_t N = (size_t)e->host[m].G.nx * e->host[m].G.ny;
+    const int nb = (int)std::min<size_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_ens_copy, dim3(nb), dim3(256), 0, s, e->host[m].buf[k.steps & 1], st.h,
+                       st.hu, st.hv, N, 1);
+    if (infos) {
+      SweRunInfo& I = infos[m];
+      I.outputs_written = k.out_idx;
+      I.steps_used = k.steps;
+      I.final_time_s = k.t;
+      I.volume_initial = k.vol0;
+      I.volume_final = k.vol;
+      I.rain_volume = k.rain;
+      I.boundary_volume = k.bnd;
+      I.clip_volume = k.clip;
+      I.mass_balance_error = k.vol - k.vol0 - k.rain - k.bnd - k.clip;
+      const double sc = std::max(std::fabs(k.vol0) + std::fabs(k.rain) + std::fabs(k.bnd) + std::fabs(k.clip), 1.0);
+      I.mass_balance_rel = I.mass_balance_error / sc;
+      I.nonfinite = k.nonfinite;
+      I._pad = 0;
+    }
+    if (k.nonfinite) rc = efail(SWE_ENONFINITE, "member " + std::to_string(m) + " went non-finite");
+    else if (!k.done && rc == SWE_OK) rc = efail(SWE_EMAXSTEPS, "member " + std::to_string(m) + " did not reach t_end");
+  }
+  ENS_TRY(hipGetLastError());
+  ENS_TRY(hipStreamSynchronize(s));
+  e->active = false;
+  return rc;
+}
+
+int swe_ensemble_run(SweEnsemble* e, double t_end, int32_t max_steps, SweRunInfo* infos, void* stream) {
+  int rc = swe_ensemble_begin(e, t_end, max_steps, stream);
+  if (rc) return rc;
+  const int chunk = e->graph_chunk > 0 ? e->graph_chunk : 16;
+  long launched = 0;
+  while (true) {
+    if ((rc = swe_ensemble_advance(e, chunk, stream))) return rc;
+    launched += chunk;
+    int running = 0;
+    if ((rc = swe_ensemble_poll(e, &running, stream))) return rc;
+    if (running == 0) break;
+    if (launched > (long)max_steps + 2L * chunk)
+      return efail(SWE_EHIP, "swe_ensemble_run: members did not stop at max_steps");
+  }
+  return swe_ensemble_end(e, infos, stream);
+}
+
+int swe_ensemble_total_waves(const SweEnsemble* e, int32_t* waves, int32_t* grid) {
+  if (!e || !waves || !grid) return efail(SWE_EINVAL, "swe_ensemble_total_waves: null argument");
+  *waves = e->total_waves;
+  *grid = e->grid;
+  return SWE_OK;
+}
+
+}  // extern "C"

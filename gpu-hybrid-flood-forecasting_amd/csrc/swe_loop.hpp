@@ -1,0 +1,192 @@
+// Device-loop control shared by the single-domain and ensemble kernels: the
+// control block each step kernel reads on entry, the FP64 partials of its
+// epilogue, and the single-thread finish that plans the next step
+// (jax_solver/numerics.py:398-418 with the float64 scalars of
+// numpy_solver/shallow_water_2d.py:267-274, 402-412).
+#pragma once
+
+#include "swe_device.hpp"
+
+namespace swe {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kDefaultRows = 16;
+
+// Device-resident state of the time loop.  The first fields are what every
+// step kernel reads on entry.
+struct LoopControl {
+  float dt_f, dtrate_f, dtg_f;  // float32(dt), float32(dt*rate), float32(dt*g)
+  int done;
+  int rec_lo, rec_hi;  // output slots the pending step writes
+  int steps, out_idx;
+  double t, dt, rate;
+  double vol0, vol, rain, bnd, clip;
+  double max_inv;
+  int nonfinite;
+  unsigned counter;  // workgroups finished in the current launch
+};
+
+struct LoopConst {
+  double cfl, max_dt, gravity, cell_area, sum_mult, rain_default, dxd, dyd;
+  const double* rain_t;
+  const double* rain_r;
+  const double* times;
+  double* dt_log;  // optional: dt of step k is written to dt_log[k] (k < dt_log_cap)
+  int n_rain, n_times, max_steps, dt_log_cap;
+};
+
+struct Partials {
+  float* inv;
+  double* vol;
+  double* clip;
+  double* bnd;
+  int* nf;
+};
+
+enum { MODE_STEP = 0, MODE_LOOP = 1 };
+enum { RED_CFL = 0, RED_INIT = 1 };
+
+template <class T>
+__device__ __forceinline__ T coherent_load(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Prepare the control block for the step that starts at t_new with the CFL
+// maximum m of the current state (numerics.py:405 with the float64 quotient
+// of numpy_solver/shallow_water_2d.py:274).  Single thread.
+__device__ void plan_next_step(LoopControl* ctl, const LoopConst& L, float m, double t_new,
+                               int oi, int rain_j) {
+  const double md = (double)m;
+  const double cfl_dt = md > 0.0 ? fmin(L.cfl / md, L.max_dt) : L.max_dt;
+  const double target = L.times[oi];
+  const double dt = fmin(cfl_dt, target - t_new);
+  const double t_end = t_new + dt;
+  int hi = oi;
+  while (hi < L.n_times && t_end >= L.times[hi] - kOutputSlack) ++hi;
+  const double rate = interp_at(L.rain_t, L.rain_r, L.n_rain, L.rain_default, t_new, rain_j);
+  ctl->dt = dt;
+  ctl->rate = rate;
+  ctl->dt_f = (float)dt;
+  ctl->dtrate_f = (float)(dt * rate);
+  ctl->dtg_f = (float)(dt * L.gravity);
+  ctl->rec_lo = oi;
+  ctl->rec_hi = hi;
+  if (L.dt_log && ctl->steps < L.dt_log_cap) L.dt_log[ctl->steps] = dt;
+}
+
+// Block-wide reduction of (max, sum, sum, sum, or) into thread 0.
+struct Totals {
+  float inv;
+  double vol, clip, bnd;
+  int nf;
+};
+
+__device__ Totals block_reduce(float inv, double vol, double clip, double bnd, int nf) {
+  __shared__ float s_inv[kWavesPerBlock];
+  __shared__ double s_vol[kWavesPerBlock], s_clip[kWavesPerBlock], s_bnd[kWavesPerBlock];
+  __shared__ int s_nf[kWavesPerBlock];
+  inv = wave_max(inv);
+  vol = wave_sum(vol);
+  clip = wave_sum(clip);
+  bnd = wave_sum(bnd);
+  nf = wave_or(nf);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_inv[w] = inv;
+    s_vol[w] = vol;
+    s_clip[w] = clip;
+    s_bnd[w] = bnd;
+    s_nf[w] = nf;
+  }
+  __syncthreads();
+  Totals t{0.0f, 0.0, 0.0, 0.0, 0};
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < kWavesPerBlock; ++k) {
+      t.inv = fmaxf(t.inv, s_inv[k]);
+      t.vol += s_vol[k];
+      t.clip += s_clip[k];
+      t.bnd += s_bnd[k];
+      t.nf |= s_nf[k];
+    }
+  }
+  __syncthreads();
+  return t;
+}
+
+// Publish this workgroup's partials; returns true (block-uniform) in the last
+// workgroup of the launch, after which every workgroup's partials are visible
+// to it (release on the counter increment, acquire in the last workgroup).
+__device__ bool publish_partials(const Partials& P, LoopControl* ctl, const Totals& t) {
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    const int b = blockIdx.x;
+    __hip_atomic_store(P.inv + b, t.inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(P.vol + b, t.vol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(P.clip + b, t.clip, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(P.bnd + b, t.bnd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(P.nf + b, t.nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned prev = __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELEASE,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return s_last != 0;
+}
+
+// Sum all partials (last workgroup only) into thread 0.
+__device__ Totals gather_partials(const Partials& P, int nparts) {
+  float inv = 0.0f;
+  double vol = 0.0, clip = 0.0, bnd = 0.0;
+  int nf = 0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    inv = fmaxf(inv, coherent_load(P.inv + k));
+    vol += coherent_load(P.vol + k);
+    clip += coherent_load(P.clip + k);
+    bnd += coherent_load(P.bnd + k);
+    nf |= coherent_load(P.nf + k);
+  }
+  return block_reduce(inv, vol, clip, bnd, nf);
+}
+
+// Index j with rain_t[j] <= t < rain_t[j+1], searched by the whole block.
+__device__ int rain_bracket(const LoopConst& L, double t) {
+  __shared__ int s_j;
+  if (threadIdx.x == 0) s_j = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < L.n_rain - 1; k += blockDim.x) {
+    if (L.rain_t[k] <= t && t < L.rain_t[k + 1]) s_j = k;
+  }
+  __syncthreads();
+  return s_j;
+}
+
+// Close the step just taken (single thread): account its rain, open-boundary
+// and clip volumes in FP64, advance the clock and step count, move past the
+// output times it reached (numerics.py:398-412), then stop or plan the next
+// step.  Resets the workgroup counter for the next launch.
+__device__ inline void finish_step(LoopControl* ctl, const LoopConst& L, const Totals& tot,
+                                   double t_new, int rain_j) {
+  ctl->rain += (double)ctl->dtrate_f * L.sum_mult * L.cell_area;
+  ctl->bnd += ctl->dt * tot.bnd;
+  ctl->clip += tot.clip * L.cell_area;
+  ctl->vol = tot.vol * L.cell_area;
+  ctl->nonfinite |= tot.nf;
+  ctl->max_inv = (double)tot.inv;
+  const int steps = ctl->steps + 1;
+  ctl->steps = steps;
+  ctl->t = t_new;
+  int oi = ctl->out_idx;
+  while (oi < L.n_times && t_new >= L.times[oi] - kOutputSlack) ++oi;
+  ctl->out_idx = oi;
+  if (oi >= L.n_times || steps >= L.max_steps || ctl->nonfinite) {
+    ctl->done = 1;
+    ctl->rec_lo = ctl->rec_hi = 0;
+  } else {
+    plan_next_step(ctl, L, tot.inv, t_new, oi, rain_j);
+  }
+  __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace swe

@@ -26,160 +26,12 @@
 #include <vector>
 
 #include "swe_device.hpp"
+#include "swe_strip.hpp"
+#include "swe_loop.hpp"
+#include "swe_host.hpp"
 #include "../../include/swe_hip.h"
 
 namespace swe {
-
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kDefaultRows = 16;
-
-// Device-resident state of the time loop.  The first fields are what every
-// step kernel reads on entry.
-struct LoopControl {
-  float dt_f, dtrate_f, dtg_f;  // float32(dt), float32(dt*rate), float32(dt*g)
-  int done;
-  int rec_lo, rec_hi;  // output slots the pending step writes
-  int steps, out_idx;
-  double t, dt, rate;
-  double vol0, vol, rain, bnd, clip;
-  double max_inv;
-  int nonfinite;
-  unsigned counter;  // workgroups finished in the current launch
-};
-
-struct LoopConst {
-  double cfl, max_dt, gravity, cell_area, sum_mult, rain_default, dxd, dyd;
-  const double* rain_t;
-  const double* rain_r;
-  const double* times;
-  int n_rain, n_times, max_steps, _pad;
-};
-
-struct Partials {
-  float* inv;
-  double* vol;
-  double* clip;
-  double* bnd;
-  int* nf;
-};
-
-enum { MODE_STEP = 0, MODE_LOOP = 1 };
-enum { RED_CFL = 0, RED_INIT = 1 };
-
-template <class T>
-__device__ __forceinline__ T coherent_load(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Prepare the control block for the step that starts at t_new with the CFL
-// maximum m of the current state (numerics.py:405 with the float64 quotient
-// of numpy_solver/shallow_water_2d.py:274).  Single thread.
-__device__ void plan_next_step(LoopControl* ctl, const LoopConst& L, float m, double t_new,
-                               int oi, int rain_j) {
-  const double md = (double)m;
-  const double cfl_dt = md > 0.0 ? fmin(L.cfl / md, L.max_dt) : L.max_dt;
-  const double target = L.times[oi];
-  const double dt = fmin(cfl_dt, target - t_new);
-  const double t_end = t_new + dt;
-  int hi = oi;
-  while (hi < L.n_times && t_end >= L.times[hi] - kOutputSlack) ++hi;
-  const double rate = interp_at(L.rain_t, L.rain_r, L.n_rain, L.rain_default, t_new, rain_j);
-  ctl->dt = dt;
-  ctl->rate = rate;
-  ctl->dt_f = (float)dt;
-  ctl->dtrate_f = (float)(dt * rate);
-  ctl->dtg_f = (float)(dt * L.gravity);
-  ctl->rec_lo = oi;
-  ctl->rec_hi = hi;
-}
-
-// Block-wide reduction of (max, sum, sum, sum, or) into thread 0.
-struct Totals {
-  float inv;
-  double vol, clip, bnd;
-  int nf;
-};
-
-__device__ Totals block_reduce(float inv, double vol, double clip, double bnd, int nf) {
-  __shared__ float s_inv[kWavesPerBlock];
-  __shared__ double s_vol[kWavesPerBlock], s_clip[kWavesPerBlock], s_bnd[kWavesPerBlock];
-  __shared__ int s_nf[kWavesPerBlock];
-  inv = wave_max(inv);
-  vol = wave_sum(vol);
-  clip = wave_sum(clip);
-  bnd = wave_sum(bnd);
-  nf = wave_or(nf);
-  const int w = threadIdx.x / kWave;
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    s_inv[w] = inv;
-    s_vol[w] = vol;
-    s_clip[w] = clip;
-    s_bnd[w] = bnd;
-    s_nf[w] = nf;
-  }
-  __syncthreads();
-  Totals t{0.0f, 0.0, 0.0, 0.0, 0};
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < kWavesPerBlock; ++k) {
-      t.inv = fmaxf(t.inv, s_inv[k]);
-      t.vol += s_vol[k];
-      t.clip += s_clip[k];
-      t.bnd += s_bnd[k];
-      t.nf |= s_nf[k];
-    }
-  }
-  __syncthreads();
-  return t;
-}
-
-// Publish this workgroup's partials; returns true (block-uniform) in the last
-// workgroup of the launch, after which every workgroup's partials are visible
-// to it (release on the counter increment, acquire in the last workgroup).
-__device__ bool publish_partials(const Partials& P, LoopControl* ctl, const Totals& t) {
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    const int b = blockIdx.x;
-    __hip_atomic_store(P.inv + b, t.inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(P.vol + b, t.vol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(P.clip + b, t.clip, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(P.bnd + b, t.bnd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(P.nf + b, t.nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned prev = __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELEASE,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return s_last != 0;
-}
-
-// Sum all partials (last workgroup only) into thread 0.
-__device__ Totals gather_partials(const Partials& P, int nparts) {
-  float inv = 0.0f;
-  double vol = 0.0, clip = 0.0, bnd = 0.0;
-  int nf = 0;
-  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
-    inv = fmaxf(inv, coherent_load(P.inv + k));
-    vol += coherent_load(P.vol + k);
-    clip += coherent_load(P.clip + k);
-    bnd += coherent_load(P.bnd + k);
-    nf |= coherent_load(P.nf + k);
-  }
-  return block_reduce(inv, vol, clip, bnd, nf);
-}
-
-// Index j with rain_t[j] <= t < rain_t[j+1], searched by the whole block.
-__device__ int rain_bracket(const LoopConst& L, double t) {
-  __shared__ int s_j;
-  if (threadIdx.x == 0) s_j = 0;
-  __syncthreads();
-  for (int k = threadIdx.x; k < L.n_rain - 1; k += blockDim.x) {
-    if (L.rain_t[k] <= t && t < L.rain_t[k + 1]) s_j = k;
-  }
-  __syncthreads();
-  return s_j;
-}
 
 // ---------------------------------------------------------------------------
 // The fused step
@@ -201,116 +53,22 @@ __global__ __launch_bounds__(kBlock) void k_step(
     rec_lo = ctl->rec_lo;
     rec_hi = ctl->rec_hi;
   }
-  const size_t N = (size_t)G.nx * (size_t)G.ny;
-  const int lane = threadIdx.x & (kWave - 1);
   const int gw = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave);
-
-  float inv_max = 0.0f;
-  double vol = 0.0, clip = 0.0, bnd = 0.0;
-  int nf = 0;
-
+  StripAcc acc;
   if (gw < n_waves) {
-    const int sx = gw % segs_x;
-    const int sy = gw / segs_x;
-    const int i = sx * kStripCols - 1 + lane;
-    const bool own = lane >= 1 && lane <= kStripCols && i < G.nx;
-    const int ic = min(max(i, 0), G.nx - 1);
-    const int j0 = sy * rows;
-    const int j1 = min(j0 + rows, G.ny);
-
-    const Cell below = load_cell<M>(G, H, HU, HV, j0 - 1, i);
-    Cell cur = load_cell<M>(G, H, HU, HV, j0, i);
-    FaceFlux fy = face_flux<M, false>(G, below.h, below.v, below.u, below.b, below.a, cur.h,
-                                      cur.v, cur.u, cur.b, cur.a);
-    // South face of the first row, seen from its north (top) side.
-    float yt0 = fy.f0, yt1 = fy.ft, yt2 = fy.fn + fy.cB;
-    if (MODE == MODE_LOOP && j0 == 0 && own) bnd += (double)fy.f0 * L.dxd;
-
-    for (int j = j0; j < j1; ++j) {
-      const Cell up = load_cell<M>(G, H, HU, HV, j + 1, i);
-      const size_t idx = (size_t)j * (size_t)G.nx + (size_t)ic;
-      const float nn = G.n[idx];
-      const float mult = G.mult[idx];
-
-      // West face of this lane's column; the lane to the west supplies its cell.
-      const float wh = from_west(cur.h), wu = from_west(cur.u), wv = from_west(cur.v);
-      const float wb = from_west(cur.b);
-      const int wa = from_west(cur.a);
-      const FaceFlux fx =
-          face_flux<M, true>(G, wh, wu, wv, wb, wa, cur.h, cur.u, cur.v, cur.b, cur.a);
-      // This cell is the east (right) side of its west face ...
-      const float xr0 = fx.f0, xr1 = fx.fn + fx.cB, xr2 = fx.ft;
-      // ... and the west (left) side of its east face, evaluated by lane+1.
-      const float xl0 = from_east(fx.f0);
-      const float xl1 = from_east(fx.fn + fx.cA);
-      const float xl2 = from_east(fx.ft);
-      if (MODE == MODE_LOOP) {
-        if (i == 0) bnd += (double)fx.f0 * L.dyd;
-        if (i == G.nx) bnd -= (double)fx.f0 * L.dyd;
-      }
-
-      // North face: this cell is its south (bottom) side.
-      const FaceFlux gy = face_flux<M, false>(G, cur.h, cur.v, cur.u, cur.b, cur.a, up.h, up.v,
-                                              up.u, up.b, up.a);
-      const float yb0 = gy.f0, yb1 = gy.ft, yb2 = gy.fn + gy.cA;
-      if (MODE == MODE_LOOP && j + 1 == G.ny && own) bnd -= (double)gy.f0 * L.dxd;
-
-      const float d0 = M::per_dx(xl0 - xr0, G.dx, G.rdx) + M::per_dx(yb0 - yt0, G.dy, G.rdy);
-      const float d1 = M::per_dx(xl1 - xr1, G.dx, G.rdx) + M::per_dx(yb1 - yt1, G.dy, G.rdy);
-      const float d2 = M::per_dx(xl2 - xr2, G.dx, G.rdx) + M::per_dx(yb2 - yt2, G.dy, G.rdy);
-      const CellUpdate r =
-          cell_update<M>(G, cur.h, cur.hu, cur.hv, d0, d1, d2, nn, mult, cur.a, dt, dtrate, dtg);
-      if (own) {
-        Ho[idx] = r.h;
-        HUo[idx] = r.hu;
-        HVo[idx] = r.hv;
-        if constexpr (MODE == MODE_LOOP) {
-          for (int k = rec_lo; k < rec_hi; ++k) {
-            cube_h[(size_t)k * N + idx] = r.h;
-            cube_hu[(size_t)k * N + idx] = r.hu;
-            cube_hv[(size_t)k * N + idx] = r.hv;
-          }
-          inv_max = fmaxf(inv_max, cfl_inverse<M>(G, r.h, r.hu, r.hv, cur.a));
-          vol += (double)r.h;
-          if (cur.a) clip += (double)fmaxf(-r.h_star, 0.0f);
-          nf |= !(isfinite(r.h) && isfinite(r.hu) && isfinite(r.hv));
-        }
-      }
-      yt0 = gy.f0;
-      yt1 = gy.ft;
-      yt2 = gy.fn + gy.cB;
-      cur = up;
-    }
+    march_strip<M, MODE == MODE_LOOP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x, rows,
+                                      dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi, cube_h,
+                                      cube_hu, cube_hv, acc);
   }
 
   if constexpr (MODE == MODE_LOOP) {
-    const Totals mine = block_reduce(inv_max, vol, clip, bnd, nf);
+    const Totals mine = block_reduce(acc.inv, acc.vol, acc.clip, acc.bnd, acc.nf);
     if (!publish_partials(P, ctl, mine)) return;
     // Last workgroup: finish the step on the device.
     const Totals tot = gather_partials(P, gridDim.x);
     const double t_new = ctl->t + ctl->dt;
     const int rj = rain_bracket(L, t_new);
-    if (threadIdx.x == 0) {
-      ctl->rain += (double)ctl->dtrate_f * L.sum_mult * L.cell_area;
-      ctl->bnd += ctl->dt * tot.bnd;
-      ctl->clip += tot.clip * L.cell_area;
-      ctl->vol = tot.vol * L.cell_area;
-      ctl->nonfinite |= tot.nf;
-      ctl->max_inv = (double)tot.inv;
-      const int steps = ctl->steps + 1;
-      ctl->steps = steps;
-      ctl->t = t_new;
-      int oi = ctl->out_idx;
-      while (oi < L.n_times && t_new >= L.times[oi] - kOutputSlack) ++oi;
-      ctl->out_idx = oi;
-      if (oi >= L.n_times || steps >= L.max_steps || ctl->nonfinite) {
-        ctl->done = 1;
-        ctl->rec_lo = ctl->rec_hi = 0;
-      } else {
-        plan_next_step(ctl, L, tot.inv, t_new, oi, rj);
-      }
-      __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) finish_step(ctl, L, tot, t_new, rj);
   }
 }
 
@@ -413,13 +171,17 @@ __global__ __launch_bounds__(kBlock) void k_copy_final(const LoopControl* __rest
 using namespace swe;
 
 namespace {
-
 thread_local std::string g_last_error;
 
-int fail(int code, const std::string& msg) {
+}  // namespace
+
+int swe::set_error(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
+
+namespace {
+int fail(int code, const std::string& msg) { return swe::set_error(code, msg); }
 
 #define SWE_HIP_TRY(expr)                                                              \
   do {                                                                                 \
@@ -450,6 +212,8 @@ struct SweContext {
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t graph = nullptr;
   std::vector<const void*> graph_key;
+  double* dt_log = nullptr;  // caller's device buffer for the dt schedule
+  int dt_log_cap = 0;
   // swe_loop_begin/advance/end state
   double loop_times[2] = {0.0, 0.0};
   LoopConst loop_L{};
@@ -477,6 +241,8 @@ struct SweContext {
     L.rain_t = d_rain;
     L.rain_r = d_rain + rain_t.size();
     L.times = d_times;
+    L.dt_log = dt_log;
+    L.dt_log_cap = dt_log_cap;
     L.n_rain = (int)rain_t.size();
     L.n_times = n_times;
     L.max_steps = max_steps;
@@ -871,6 +637,19 @@ int swe_set_graph_chunk(SweContext* c, int32_t chunk) {
   if (chunk < 0 || chunk > 4096 || (chunk & 1))
     return fail(SWE_EINVAL, "swe_set_graph_chunk: chunk must be an even number in [0, 4096]");
   c->graph_chunk = chunk;
+  return SWE_OK;
+}
+
+int swe_set_dt_log(SweContext* c, double* dt_log, int32_t capacity) {
+  if (!c || capacity < 0 || (capacity > 0 && !dt_log))
+    return fail(SWE_EINVAL, "swe_set_dt_log: bad arguments");
+  c->dt_log = capacity > 0 ? dt_log : nullptr;
+  c->dt_log_cap = capacity;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    c->graph_key.clear();
+  }
   return SWE_OK;
 }
 

@@ -77,6 +77,10 @@ class SweRunInfo(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "_pad"}
 
 
+class SweMemberDesc(ctypes.Structure):
+    _fields_ = [("params", SweParamsDesc), ("state", SweState)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/swe_hip.h
 SIGNATURES = {
     "swe_version": (c_char_p, []),
@@ -86,6 +90,7 @@ SIGNATURES = {
     "swe_set_variant": (c_int32, [c_void_p, c_int32]),
     "swe_set_tile_rows": (c_int32, [c_void_p, c_int32]),
     "swe_set_graph_chunk": (c_int32, [c_void_p, c_int32]),
+    "swe_set_dt_log": (c_int32, [c_void_p, c_void_p, c_int32]),
     "swe_step": (c_int32, [c_void_p, SweState, SweState, c_double, c_double, c_void_p]),
     "swe_cfl_dt": (c_int32, [c_void_p, SweState, POINTER(c_double), c_void_p]),
     "swe_cfl_field": (c_int32, [c_void_p, SweState, c_void_p, c_void_p]),
@@ -105,6 +110,16 @@ SIGNATURES = {
     "swe_loop_begin": (c_int32, [c_void_p, SweState, c_double, c_int32, c_void_p]),
     "swe_loop_advance": (c_int32, [c_void_p, c_int32, c_void_p]),
     "swe_loop_end": (c_int32, [c_void_p, SweState, POINTER(SweRunInfo), c_void_p]),
+    "swe_ensemble_create": (c_int32, [POINTER(SweMemberDesc), c_int32, c_int32, POINTER(c_void_p)]),
+    "swe_ensemble_destroy": (c_int32, [c_void_p]),
+    "swe_ensemble_set_tile_rows": (c_int32, [c_void_p, c_int32]),
+    "swe_ensemble_set_graph_chunk": (c_int32, [c_void_p, c_int32]),
+    "swe_ensemble_begin": (c_int32, [c_void_p, c_double, c_int32, c_void_p]),
+    "swe_ensemble_advance": (c_int32, [c_void_p, c_int32, c_void_p]),
+    "swe_ensemble_poll": (c_int32, [c_void_p, POINTER(c_int32), c_void_p]),
+    "swe_ensemble_end": (c_int32, [c_void_p, POINTER(SweRunInfo), c_void_p]),
+    "swe_ensemble_run": (c_int32, [c_void_p, c_double, c_int32, POINTER(SweRunInfo), c_void_p]),
+    "swe_ensemble_total_waves": (c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
 }
 
 _lib = None

@@ -174,9 +174,52 @@ int swe_loop_begin(SweContext* ctx, SweState state, double t0, int32_t max_steps
 int swe_loop_advance(SweContext* ctx, int32_t n_launches, void* stream);
 int swe_loop_end(SweContext* ctx, SweState state, SweRunInfo* info, void* stream);
 
+/* Record the device loop's step sizes: the float64 dt of step k (counted
+ * from the loop's start) is written to dt_log[k] for k < capacity.  The
+ * schedule lets a caller replay the exact same steps with swe_step (the
+ * adjoint recomputes its checkpoints that way).  capacity 0 turns it off. */
+int swe_set_dt_log(SweContext* ctx, double* dt_log, int32_t capacity);
+
 /* Use HIP graphs of `chunk` captured step launches for the device loop
  * (0 = plain launches).  Default 64. */
 int swe_set_graph_chunk(SweContext* ctx, int32_t chunk);
+
+/* ------------------------------------------------------------------------
+ * Batched ensemble (no counterpart in the reference, which runs one domain;
+ * it runs many independent rainfall/roughness scenarios of the same step).
+ * Each member is a full domain of its own size with its own fields, rain
+ * series and controls.  One persistent launch advances every unfinished
+ * member by one of its own CFL steps; chunks of launches are captured in a
+ * HIP graph.
+ * ------------------------------------------------------------------------ */
+typedef struct SweEnsemble SweEnsemble;
+
+typedef struct SweMemberDesc {
+  SweParamsDesc params; /* as for swe_create; device fields stay valid for the ensemble's life */
+  SweState state;       /* device: initial state in (read by swe_ensemble_begin), final state out */
+} SweMemberDesc;
+
+int swe_ensemble_create(const SweMemberDesc* members, int32_t n_members, int32_t variant,
+                        SweEnsemble** out);
+int swe_ensemble_destroy(SweEnsemble* ens);
+/* Rows per wavefront strip for every member (tuning knob). */
+int swe_ensemble_set_tile_rows(SweEnsemble* ens, int32_t rows);
+/* Launches per captured HIP graph (0 = plain launches). */
+int swe_ensemble_set_graph_chunk(SweEnsemble* ens, int32_t chunk);
+/* Copy every member's state in and prime its clock at 0 with target t_end. */
+int swe_ensemble_begin(SweEnsemble* ens, double t_end, int32_t max_steps, void* stream);
+/* Queue n_launches ensemble steps (rounded up to the graph chunk). */
+int swe_ensemble_advance(SweEnsemble* ens, int32_t n_launches, void* stream);
+/* Synchronise and count members still running. */
+int swe_ensemble_poll(SweEnsemble* ens, int32_t* running, void* stream);
+/* Copy final states out; fill infos[n_members]; SWE_EMAXSTEPS if a member
+ * did not reach t_end, SWE_ENONFINITE if one went non-finite.  Synchronises. */
+int swe_ensemble_end(SweEnsemble* ens, SweRunInfo* infos, void* stream);
+/* begin + advance/poll until every member reaches t_end + end. */
+int swe_ensemble_run(SweEnsemble* ens, double t_end, int32_t max_steps, SweRunInfo* infos,
+                     void* stream);
+/* Strips (wavefronts of work) per launch and the persistent grid size. */
+int swe_ensemble_total_waves(const SweEnsemble* ens, int32_t* waves, int32_t* grid);
 
 #ifdef __cplusplus
 }
