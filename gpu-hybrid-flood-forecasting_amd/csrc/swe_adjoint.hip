@@ -1,0 +1,223 @@
+// Adjoint (reverse) of one fused step, by hand: given the state U_n entering
+// step n and the cotangent lambda_{n+1} of the state leaving it, produce
+// lambda_n = (dU_{n+1}/dU_n)^T lambda_{n+1} and accumulate the cell-wise
+// sensitivities to the Manning field and to the rain multiplier.  The step
+// size is data (the recorded schedule of the forward run).
+//
+// One wavefront owns 60 columns (lanes 2..61; two halo lanes each side,
+// because a cell's lambda needs both its faces' reverse, and a face's
+// reverse needs the pre-source cotangent lambda* of both its cells, which in
+// turn needs that cell's two x faces).  It marches north: at row r it
+// recomputes the forward of row r, reverses the pointwise part into
+// lambda*_r, reverses its west x face (the west cell's share leaves by a DPP
+// wave shift) and the y face r-1/2 below it, which completes row r-1.
+// Cotangents that land on ghost cells are folded back into the edge cell
+// they copy (negated normal momentum on reflective sides).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/swe_hip.h"
+#include "swe_adjoint.hpp"
+#include "swe_device.hpp"
+#include "swe_host.hpp"
+#include "swe_strip.hpp"
+
+namespace swe {
+
+constexpr int kAdjCols = kWave - 4;  // columns completed per wavefront
+
+template <class M>
+__global__ __launch_bounds__(256) void k_adjoint_step(
+    Grid G, const float* __restrict__ H, const float* __restrict__ HU,
+    const float* __restrict__ HV, const float* __restrict__ LH, const float* __restrict__ LHU,
+    const float* __restrict__ LHV, float* __restrict__ OH, float* __restrict__ OHU,
+    float* __restrict__ OHV, float* __restrict__ grad_n, float* __restrict__ grad_mult, float dt,
+    float dtrate, float dtg, int rows, int segs_x, int n_waves) {
+  const int gw = blockIdx.x * (blockDim.x / kWave) + (int)(threadIdx.x / kWave);
+  if (gw >= n_waves) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int sx = gw % segs_x, sy = gw / segs_x;
+  const int i = sx * kAdjCols - 2 + lane;
+  const bool real = i >= 0 && i < G.nx;
+  const bool own = lane >= 2 && lane <= kAdjCols + 1 && i < G.nx;
+  const bool refl_w = G.bt[0] == 0, refl_e = G.bt[1] == 0;
+  const bool refl_s = G.bt[2] == 0, refl_n = G.bt[3] == 0;
+  ColumnReader<M> rd;
+  rd.H = H;
+  rd.HU = HU;
+  rd.HV = HV;
+  rd.ic = min(max(i, 0), G.nx - 1);
+  rd.neg_hu = (i < 0 && refl_w) || (i >= G.nx && refl_e);
+  rd.nx = G.nx;
+  rd.ny = G.ny;
+  rd.refl_south = refl_s;
+  rd.refl_north = refl_n;
+  const int j0 = sy * rows;
+  const int j1 = min(j0 + rows, G.ny);
+  const float cx = dt * G.rdx, cy = dt * G.rdy;
+  const float g = G.g, hg = G.half_g, dry = G.dry;
+
+  Cell Rprev = rd.load(G, j0 - 2);
+  Cell Rcur = rd.load(G, j0 - 1);
+  FaceFlux Gs = face_flux<M, false>(G, Rprev.h, Rprev.v, Rprev.u, Rprev.b, Rprev.a, Rcur.h, Rcur.v,
+                                    Rcur.u, Rcur.b, Rcur.a);
+  float lsp0 = 0.0f, lsp1 = 0.0f, lsp2 = 0.0f;  // lambda* of row r-1
+  float lup0 = 0.0f, lup1 = 0.0f, lup2 = 0.0f;  // lambda of row r-1 (partial)
+
+  for (int r = j0 - 1; r <= j1; ++r) {
+    const bool valid_row = r >= 0 && r < G.ny;
+    const Cell Rnext = rd.load(G, r + 1);
+    float ls0 = 0.0f, ls1 = 0.0f, ls2 = 0.0f;
+    float lu0 = 0.0f, lu1 = 0.0f, lu2 = 0.0f;
+    FaceFlux Gn = Gs;
+    if (valid_row) {
+      const size_t idx = (size_t)r * (size_t)G.nx + (size_t)rd.ic;
+      // ---- forward of row r (as the step kernel computes it) ----
+      Gn = face_flux<M, false>(G, Rcur.h, Rcur.v, Rcur.u, Rcur.b, Rcur.a, Rnext.h, Rnext.v,
+                               Rnext.u, Rnext.b, Rnext.a);
+      const float wh = from_west(Rcur.h), wu = from_west(Rcur.u), wv = from_west(Rcur.v);
+      const float wb = from_west(Rcur.b);
+      const float whu = from_west(Rcur.hu), whv = from_west(Rcur.hv);
+      const int wa = from_west(Rcur.a);
+      const FaceFlux Fx = face_flux<M, true>(G, wh, wu, wv, wb, wa, Rcur.h, Rcur.u, Rcur.v,
+                                             Rcur.b, Rcur.a);
+      const float xr0 = Fx.f0, xr1 = Fx.fn + Fx.cB, xr2 = Fx.ft;
+      const float xl0 = from_east(Fx.f0), xl1 = from_east(Fx.fn + Fx.cA), xl2 = from_east(Fx.ft);
+      const float yb0 = Gn.f0, yb1 = Gn.ft, yb2 = Gn.fn + Gn.cA;
+      const float yt0 = Gs.f0, yt1 = Gs.ft, yt2 = Gs.fn + Gs.cB;
+      const float d0 = M::per_dx(xl0 - xr0, G.dx, G.rdx) + M::per_dx(yb0 - yt0, G.dy, G.rdy);
+      const float d1 = M::per_dx(xl1 - xr1, G.dx, G.rdx) + M::per_dx(yb1 - yt1, G.dy, G.rdy);
+      const float d2 = M::per_dx(xl2 - xr2, G.dx, G.rdx) + M::per_dx(yb2 - yt2, G.dy, G.rdy);
+      float hs = Rcur.h - dt * d0;
+      const float hus = Rcur.hu - dt * d1;
+      const float hvs = Rcur.hv - dt * d2;
+      const float mult = G.mult[idx];
+      hs = hs + dtrate * mult;
+      // ---- reverse of the pointwise part ----
+      const adj::PointVjp<float> pv =
+          adj::point_vjp<float>(dry, kPow73, hs, hus, hvs, G.n[idx], Rcur.a, dtrate, dtg,
+                                LH[idx], LHU[idx], LHV[idx]);
+      if (real) {
+        ls0 = pv.lh;
+        ls1 = pv.lhu;
+        ls2 = pv.lhv;
+      }
+      if (own && r >= j0 && r < j1) {
+        if (grad_n) grad_n[idx] += pv.gn;
+        if (grad_mult) grad_mult[idx] += pv.gmult;
+      }
+      lu0 = ls0;
+      lu1 = ls1;
+      lu2 = ls2;
+      // ---- reverse of this lane's west x face (A = west cell, B = this cell) ----
+      const float lw0 = from_west(ls0), lw1 = from_west(ls1), lw2 = from_west(ls2);
+      adj::SideGrad<float> dA, dB;
+      adj::face_vjp<true, float>(g, hg, dry, wh, whu, whv, wb, wa, Rcur.h, Rcur.hu, Rcur.hv,
+                                 Rcur.b, Rcur.a, cx * (ls0 - lw0), cx * (ls1 - lw1),
+                                 cx * (ls2 - lw2), -cx * lw1, cx * ls1, dA, dB);
+      // x: (h, q_n, q_t) = (h, hu, hv)
+      lu0 += dB.h;
+      lu1 += dB.qn;
+      lu2 += dB.qt;
+      float sw0 = dA.h, sw1 = dA.qn, sw2 = dA.qt;
+      if (i == 0) {  // west cell is the left ghost: it copies this cell
+        lu0 += dA.h;
+        lu1 += refl_w ? -dA.qn : dA.qn;
+        lu2 += dA.qt;
+        sw0 = sw1 = sw2 = 0.0f;
+      }
+      if (i == G.nx) {  // this lane is the right ghost: its share belongs to cell nx-1
+        sw0 += dB.h;
+        sw1 += refl_e ? -dB.qn : dB.qn;
+        sw2 += dB.qt;
+      }
+      lu0 += from_east(sw0);
+      lu1 += from_east(sw1);
+      lu2 += from_east(sw2);
+    }
+    // ---- reverse of the y face r-1/2 (A = row r-1, B = row r) ----
+    if (r >= j0) {
+      adj::SideGrad<float> dA, dB;
+      // y: normal momentum hv, tangential hu
+      adj::face_vjp<false, float>(g, hg, dry, Rprev.h, Rprev.hv, Rprev.hu, Rprev.b, Rprev.a,
+                                  Rcur.h, Rcur.hv, Rcur.hu, Rcur.b, Rcur.a, cy * (ls0 - lsp0),
+                                  cy * (ls2 - lsp2), cy * (ls1 - lsp1), -cy * lsp2, cy * ls2, dA,
+                                  dB);
+      if (r - 1 < 0) {  // row r-1 is the south ghost: it copies row 0 (= row r)
+        lu0 += dA.h;
+        lu1 += dA.qt;
+        lu2 += refl_s ? -dA.qn : dA.qn;
+      } else {
+        lup0 += dA.h;
+        lup1 += dA.qt;
+        lup2 += dA.qn;
+      }
+      if (r >= G.ny) {  // row r is the north ghost: it copies row ny-1 (= row r-1)
+        lup0 += dB.h;
+        lup1 += dB.qt;
+        lup2 += refl_n ? -dB.qn : dB.qn;
+      } else {
+        lu0 += dB.h;
+        lu1 += dB.qt;
+        lu2 += dB.qn;
+      }
+      if (own && r - 1 >= j0 && r - 1 < j1) {
+        const size_t idp = (size_t)(r - 1) * (size_t)G.nx + (size_t)rd.ic;
+        OH[idp] = lup0;
+        OHU[idp] = lup1;
+        OHV[idp] = lup2;
+      }
+    }
+    Gs = Gn;
+    Rprev = Rcur;
+    Rcur = Rnext;
+    lsp0 = ls0;
+    lsp1 = ls1;
+    lsp2 = ls2;
+    lup0 = lu0;
+    lup1 = lu1;
+    lup2 = lu2;
+  }
+}
+
+}  // namespace swe
+
+using namespace swe;
+
+namespace {
+constexpr int kAdjRows = 16;
+}
+
+extern "C" int swe_adjoint_step(SweContext* c, SweState state, SweState lam_next, SweState lam_out,
+                                double dt, double time_s, float* grad_manning, float* grad_mult,
+                                void* stream) {
+  if (!c || !state.h || !state.hu || !state.hv || !lam_next.h || !lam_next.hu || !lam_next.hv ||
+      !lam_out.h || !lam_out.hu || !lam_out.hv)
+    return set_error(SWE_EINVAL, "swe_adjoint_step: null argument");
+  if (lam_out.h == lam_next.h || lam_out.hu == lam_next.hu || lam_out.hv == lam_next.hv)
+    return set_error(SWE_EINVAL, "swe_adjoint_step: lam_out must not alias lam_next");
+  const Grid& G = *context_grid(c);
+  const double rate = context_rate(c, time_s);
+  const float dt_f = (float)dt, dtrate_f = (float)(dt * rate);
+  const float dtg_f = (float)(dt * context_gravity(c));
+  const int segs_x = (G.nx + kAdjCols - 1) / kAdjCols;
+  const int n_waves = segs_x * ((G.ny + kAdjRows - 1) / kAdjRows);
+  const int blocks = (n_waves + 3) / 4;
+  hipStream_t s = (hipStream_t)stream;
+  if (context_variant(c) == SWE_VARIANT_REFERENCE)
+    hipLaunchKernelGGL(k_adjoint_step<RefMath>, dim3(blocks), dim3(256), 0, s, G, state.h,
+                       state.hu, state.hv, lam_next.h, lam_next.hu, lam_next.hv, lam_out.h,
+                       lam_out.hu, lam_out.hv, grad_manning, grad_mult, dt_f, dtrate_f, dtg_f,
+                       kAdjRows, segs_x, n_waves);
+  else
+    hipLaunchKernelGGL(k_adjoint_step<FastMath>, dim3(blocks), dim3(256), 0, s, G, state.h,
+                       state.hu, state.hv, lam_next.h, lam_next.hu, lam_next.hv, lam_out.h,
+                       lam_out.hu, lam_out.hv, grad_manning, grad_mult, dt_f, dtrate_f, dtg_f,
+                       kAdjRows, segs_x, n_waves);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(SWE_EHIP, std::string("swe_adjoint_step: ") + hipGetErrorString(e));
+  return SWE_OK;
+}

@@ -534,7 +534,8 @@ int swe_ensemble_end(SweEnsemble* e, SweRunInfo* infos, void* stream) {
       I._pad = 0;
     }
     if (k.nonfinite) rc = efail(SWE_ENONFINITE, "member " + std::to_string(m) + " went non-finite");
-    else if (!k.done && rc == SWE_OK) rc = efail(SWE_EMAXSTEPS, "member " + std::to_string(m) + " did not reach t_end");
+    else if (k.out_idx < 2 && rc == SWE_OK)
+      rc = efail(SWE_EMAXSTEPS, "member " + std::to_string(m) + " did not reach t_end");
   }
   ENS_TRY(hipGetLastError());
   ENS_TRY(hipStreamSynchronize(s));

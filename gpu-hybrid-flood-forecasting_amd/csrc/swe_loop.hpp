@@ -44,7 +44,10 @@ struct Partials {
   int* nf;
 };
 
-enum { MODE_STEP = 0, MODE_LOOP = 1 };
+// MODE_STEP: caller-given dt, no epilogue.  MODE_LOOP: device loop, the last
+// workgroup finishes the step.  MODE_LOOP_SPLIT: device loop, every
+// workgroup stores its partials and a one-workgroup kernel finishes the step.
+enum { MODE_STEP = 0, MODE_LOOP = 1, MODE_LOOP_SPLIT = 2 };
 enum { RED_CFL = 0, RED_INIT = 1 };
 
 template <class T>

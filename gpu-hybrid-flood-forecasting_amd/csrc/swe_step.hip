@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
     float* __restrict__ cube_h, float* __restrict__ cube_hu, float* __restrict__ cube_hv) {
   float dt = dt_arg, dtrate = dtrate_arg, dtg = dtg_arg;
   int rec_lo = 0, rec_hi = 0;
-  if constexpr (MODE == MODE_LOOP) {
+  if constexpr (MODE != MODE_STEP) {
     if (ctl->done) return;
     dt = ctl->dt_f;
     dtrate = ctl->dtrate_f;
@@ -56,11 +56,22 @@ __global__ __launch_bounds__(kBlock) void k_step(
   const int gw = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave);
   StripAcc acc;
   if (gw < n_waves) {
-    march_strip<M, MODE == MODE_LOOP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x, rows,
+    march_strip<M, MODE != MODE_STEP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x, rows,
                                       dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi, cube_h,
                                       cube_hu, cube_hv, acc);
   }
 
+  if constexpr (MODE == MODE_LOOP_SPLIT) {
+    const Totals mine = block_reduce(acc.inv, acc.vol, acc.clip, acc.bnd, acc.nf);
+    if (threadIdx.x == 0) {
+      const int b = blockIdx.x;
+      P.inv[b] = mine.inv;
+      P.vol[b] = mine.vol;
+      P.clip[b] = mine.clip;
+      P.bnd[b] = mine.bnd;
+      P.nf[b] = mine.nf;
+    }
+  }
   if constexpr (MODE == MODE_LOOP) {
     const Totals mine = block_reduce(acc.inv, acc.vol, acc.clip, acc.bnd, acc.nf);
     if (!publish_partials(P, ctl, mine)) return;
@@ -75,6 +86,27 @@ __global__ __launch_bounds__(kBlock) void k_step(
 // ---------------------------------------------------------------------------
 // CFL maximum / volume of a state; MODE RED_INIT primes the loop.
 // ---------------------------------------------------------------------------
+// Finish of MODE_LOOP_SPLIT: one workgroup reduces the step kernel's
+// partials (visible at the kernel boundary) and plans the next step.
+__global__ __launch_bounds__(kBlock) void k_finish(LoopControl* __restrict__ ctl, LoopConst L,
+                                                  Partials P, int nparts) {
+  if (ctl->done) return;
+  float inv = 0.0f;
+  double vol = 0.0, clip = 0.0, bnd = 0.0;
+  int nf = 0;
+  for (int k = threadIdx.x; k < nparts; k += kBlock) {
+    inv = fmaxf(inv, P.inv[k]);
+    vol += P.vol[k];
+    clip += P.clip[k];
+    bnd += P.bnd[k];
+    nf |= P.nf[k];
+  }
+  const Totals tot = block_reduce(inv, vol, clip, bnd, nf);
+  const double t_new = ctl->t + ctl->dt;
+  const int rj = rain_bracket(L, t_new);
+  if (threadIdx.x == 0) finish_step(ctl, L, tot, t_new, rj);
+}
+
 template <class M, int MODE>
 __global__ __launch_bounds__(kBlock) void k_reduce(Grid G, const float* __restrict__ H,
                                                   const float* __restrict__ HU,
@@ -212,6 +244,7 @@ struct SweContext {
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t graph = nullptr;
   std::vector<const void*> graph_key;
+  int split_finish = 1;      // finish each loop step in its own one-workgroup kernel
   double* dt_log = nullptr;  // caller's device buffer for the dt schedule
   int dt_log_cap = 0;
   // swe_loop_begin/advance/end state
@@ -302,6 +335,16 @@ void launch_step(SweContext* c, const float* h, const float* hu, const float* hv
 void launch_step_any(SweContext* c, int mode, const float* h, const float* hu, const float* hv,
                      float* ho, float* huo, float* hvo, float dt, float dtrate, float dtg,
                      const LoopConst& L, float* ch, float* chu, float* chv, hipStream_t s) {
+  if (mode == MODE_LOOP && c->split_finish) {
+    if (c->variant == SWE_VARIANT_REFERENCE)
+      launch_step<RefMath, MODE_LOOP_SPLIT>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch,
+                                            chu, chv, s);
+    else
+      launch_step<FastMath, MODE_LOOP_SPLIT>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch,
+                                             chu, chv, s);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, s, c->d_ctl, L, c->P, c->step_blocks());
+    return;
+  }
   if (c->variant == SWE_VARIANT_REFERENCE) {
     if (mode == MODE_LOOP)
       launch_step<RefMath, MODE_LOOP>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch, chu,
@@ -359,7 +402,8 @@ int launch_loop_chunk(SweContext* c, int count, const LoopConst& L, float* ch, f
                                     (const void*)(intptr_t)c->rows,
                                     (const void*)(intptr_t)c->graph_chunk, c->d_times,
                                     (const void*)(intptr_t)L.n_times,
-                                    (const void*)(intptr_t)L.max_steps};
+                                    (const void*)(intptr_t)L.max_steps,
+                                    (const void*)(intptr_t)c->split_finish};
     if (!c->graph || key != c->graph_key) {
       if (c->graph) {
         SWE_HIP_TRY(hipGraphExecDestroy(c->graph));
@@ -485,6 +529,12 @@ int copy_state(SweContext* c, float* dh, float* dhu, float* dhv, const float* sh
 }
 
 }  // namespace
+
+// Accessors for the other translation units (the adjoint).
+const Grid* swe::context_grid(const SweContext* c) { return &c->G; }
+int swe::context_variant(const SweContext* c) { return c->variant; }
+double swe::context_gravity(const SweContext* c) { return c->gravity; }
+double swe::context_rate(const SweContext* c, double t) { return host_interp(c, t); }
 
 // ===========================================================================
 // C ABI
@@ -637,6 +687,17 @@ int swe_set_graph_chunk(SweContext* c, int32_t chunk) {
   if (chunk < 0 || chunk > 4096 || (chunk & 1))
     return fail(SWE_EINVAL, "swe_set_graph_chunk: chunk must be an even number in [0, 4096]");
   c->graph_chunk = chunk;
+  return SWE_OK;
+}
+
+int swe_set_loop_finish(SweContext* c, int32_t split) {
+  if (!c || (split != 0 && split != 1)) return fail(SWE_EINVAL, "swe_set_loop_finish: split must be 0 or 1");
+  c->split_finish = split;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    c->graph_key.clear();
+  }
   return SWE_OK;
 }
 

@@ -91,6 +91,9 @@ SIGNATURES = {
     "swe_set_tile_rows": (c_int32, [c_void_p, c_int32]),
     "swe_set_graph_chunk": (c_int32, [c_void_p, c_int32]),
     "swe_set_dt_log": (c_int32, [c_void_p, c_void_p, c_int32]),
+    "swe_set_loop_finish": (c_int32, [c_void_p, c_int32]),
+    "swe_adjoint_step": (c_int32, [c_void_p, SweState, SweState, SweState, c_double, c_double,
+                                   c_void_p, c_void_p, c_void_p]),
     "swe_step": (c_int32, [c_void_p, SweState, SweState, c_double, c_double, c_void_p]),
     "swe_cfl_dt": (c_int32, [c_void_p, SweState, POINTER(c_double), c_void_p]),
     "swe_cfl_field": (c_int32, [c_void_p, SweState, c_void_p, c_void_p]),
@@ -219,6 +222,9 @@ class Context:
 
     def set_tile_rows(self, rows: int) -> None:
         check(load().swe_set_tile_rows(self.handle, int(rows)), "swe_set_tile_rows")
+
+    def set_loop_finish(self, split: int) -> None:
+        check(load().swe_set_loop_finish(self.handle, int(split)), "swe_set_loop_finish")
 
     def set_graph_chunk(self, chunk: int) -> None:
         check(load().swe_set_graph_chunk(self.handle, int(chunk)), "swe_set_graph_chunk")

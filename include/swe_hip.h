@@ -108,6 +108,11 @@ int swe_destroy(SweContext* ctx);
 int swe_set_variant(SweContext* ctx, int32_t variant);
 /* Rows per wavefront strip of the fused step (tuning knob; 0 = default). */
 int swe_set_tile_rows(SweContext* ctx, int32_t rows);
+/* How the device loop closes a step (tuning knob): 1 (default) = every
+ * workgroup stores its CFL/mass partials and a one-workgroup kernel reduces
+ * them and plans the next step; 0 = the last workgroup of the step kernel does
+ * it (one launch per step, but a release/acquire across workgroups). */
+int swe_set_loop_finish(SweContext* ctx, int32_t split);
 
 /* One step with caller-given dt and time (finite_volume_step,
  * jax_solver/numerics.py:312-369).  `in` and `out` must not alias. */
@@ -183,6 +188,18 @@ int swe_set_dt_log(SweContext* ctx, double* dt_log, int32_t capacity);
 /* Use HIP graphs of `chunk` captured step launches for the device loop
  * (0 = plain launches).  Default 64. */
 int swe_set_graph_chunk(SweContext* ctx, int32_t chunk);
+
+/* Adjoint of one step (no counterpart in the reference: it never
+ * differentiates its solver; this is the reverse of finite_volume_step,
+ * jax_solver/numerics.py:312-369, with dt treated as data).  Given the state
+ * `state` entering a step of size dt at time_s and the cotangent `lam_next`
+ * of the state it produces, writes the cotangent `lam_out` of `state` and
+ * ADDS this step's sensitivities to the per-cell Manning field and rain
+ * multiplier into grad_manning / grad_mult (either may be NULL).  lam_out
+ * must not alias lam_next.  Asynchronous on `stream`. */
+int swe_adjoint_step(SweContext* ctx, SweState state, SweState lam_next, SweState lam_out,
+                     double dt, double time_s, float* grad_manning, float* grad_mult,
+                     void* stream);
 
 /* ------------------------------------------------------------------------
  * Batched ensemble (no counterpart in the reference, which runs one domain;

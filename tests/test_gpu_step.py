@@ -127,7 +127,7 @@ def test_step_reference_variant_bit_exact_depth(cuda, shape, sides):
     got = [x.cpu().numpy() for x in out]
     assert np.array_equal(got[0], ref.h), f"depth differs at {np.argwhere(got[0] != ref.h)[:5]}"
     for g, r in zip(got[1:], ref[1:]):
-        assert int(_ulp_diff(g, r).max()) <= 2
+        assert int(_ulp_diff(g, r).max()) <= 4
     assert nm.compute_cfl_dt(_state(case, cuda), params, variant="reference") == dt
 
 
