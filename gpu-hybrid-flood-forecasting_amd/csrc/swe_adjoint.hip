@@ -72,12 +72,13 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
     const Cell Rnext = rd.load(G, r + 1);
     float ls0 = 0.0f, ls1 = 0.0f, ls2 = 0.0f;
     float lu0 = 0.0f, lu1 = 0.0f, lu2 = 0.0f;
-    FaceFlux Gn = Gs;
+    // Forward flux of the face r+1/2; needed as row r+1's south face even when
+    // row r itself is the south ghost.
+    const FaceFlux Gn = face_flux<M, false>(G, Rcur.h, Rcur.v, Rcur.u, Rcur.b, Rcur.a, Rnext.h,
+                                            Rnext.v, Rnext.u, Rnext.b, Rnext.a);
     if (valid_row) {
       const size_t idx = (size_t)r * (size_t)G.nx + (size_t)rd.ic;
       // ---- forward of row r (as the step kernel computes it) ----
-      Gn = face_flux<M, false>(G, Rcur.h, Rcur.v, Rcur.u, Rcur.b, Rcur.a, Rnext.h, Rnext.v,
-                               Rnext.u, Rnext.b, Rnext.a);
       const float wh = from_west(Rcur.h), wu = from_west(Rcur.u), wv = from_west(Rcur.v);
       const float wb = from_west(Rcur.b);
       const float whu = from_west(Rcur.hu), whv = from_west(Rcur.hv);

@@ -69,19 +69,20 @@ class OracleParams:
     dry_tolerance: float = 1.0e-4
     max_dt: float = 10.0
     dtype: type = np.float32
+    # Fields are stored in float32 first (shallow_water_2d.py:237-251), then
+    # widened for the float64 oracle, so both oracles see identical data.
+    # Finite-difference checks pass np.float64 here to keep tiny perturbations.
+    field_dtype: type = np.float32
 
     def __post_init__(self) -> None:
         dt = self.dtype
-        # Inputs are stored as float32 first (shallow_water_2d.py:237-251), then
-        # widened for the float64 oracle so both oracles see identical data.
-        self.bed = np.asarray(np.asarray(self.bed, np.float32), dt)
-        self.manning_n = np.asarray(np.asarray(self.manning_n, np.float32), dt)
+        fd = self.field_dtype
+        self.bed = np.asarray(np.asarray(self.bed, fd), dt)
+        self.manning_n = np.asarray(np.asarray(self.manning_n, fd), dt)
         self.domain_mask = np.asarray(self.domain_mask, bool)
         if self.rainfall_multiplier is None:  # shallow_water_2d.py:229-233
             self.rainfall_multiplier = self.domain_mask.astype(np.float32)
-        self.rainfall_multiplier = np.asarray(
-            np.asarray(self.rainfall_multiplier, np.float32), dt
-        )
+        self.rainfall_multiplier = np.asarray(np.asarray(self.rainfall_multiplier, fd), dt)
         self.rainfall_times_s = np.asarray(self.rainfall_times_s, np.float32)
         self.rainfall_rates_m_s = np.asarray(self.rainfall_rates_m_s, np.float32)
         if self.dy is None:
