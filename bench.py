@@ -4,8 +4,12 @@ Workload: the 4096 x 4096 FP32 urban dam-break case (dx = 1 m, wet start
 h = 0.05 m with a 0.5 m square in the centre, no rain, cfl 0.45, one open edge),
 advanced by the device-resident adaptive loop: each step is one fused kernel
 (ghosts, reconstruction, HLL, rain, clip, friction, wet/dry mask) whose
-epilogue reduces the CFL maximum and FP64 mass partials and whose last
-workgroup sets the next dt.  Inputs are resident in HBM before timing starts.
+epilogue writes per-workgroup CFL maxima and FP64 mass partials, followed by a
+one-workgroup finish kernel that sets the next dt; chunks of steps are HIP
+graphs.  Inputs are resident in HBM before timing starts.
+
+Also reported: the batched ensemble (64 uneven urban members, 3600 s each)
+and the checkpointed adjoint (1024^2, ~2000 steps) against the plain step.
 
 Prints ONE JSON line.  ``value`` = cell-updates/s over exactly ``--steps``
 timed steps (HIP events on the stream the kernels run on, bracketed by
@@ -105,6 +109,119 @@ def cpu_baseline(case, steps):
     }
 
 
+def bench_ensemble(variant: str, count: int, seed: int, t_end: float, plain_cups: float):
+    """Config 4: `count` urban members of uneven size, n x U[0.7,1.3], rain x U[0.5,2]."""
+    import torch
+
+    from hybrid_flood_amd import synthetic as sy
+    from hybrid_flood_amd.autotune import tune_ensemble_rows
+    from hybrid_flood_amd.ensemble import EnsembleRunner, member_from_case
+
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    members = []
+    sides = []
+    for side, mseed, n_scale, rain_scale in sy.ensemble_members(count, seed):
+        case = sy.urban_case(side, dx=2.0, seed=mseed, duration_s=t_end)
+        members.append(member_from_case(case, device=dev, n_scale=n_scale, rain_scale=rain_scale))
+        sides.append(side)
+    gen_s = time.perf_counter() - t0
+    runner = EnsembleRunner(members, variant=variant)
+    tune = tune_ensemble_rows(runner)
+    chunk = 32
+    runner.set_graph_chunk(chunk)
+    torch.cuda.synchronize(dev)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    ev0.record()
+    runner.begin(t_end, max_steps=200_000)
+    launches = 0
+    while True:
+        runner.advance(chunk)
+        launches += chunk
+        if runner.running() == 0:
+            break
+    ev1.record()
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - w0
+    rc, infos = runner.end()
+    ms = ev0.elapsed_time(ev1)
+    cell_steps = sum(m.params.bed.numel() * i["steps_used"] for m, i in zip(members, infos))
+    cups = cell_steps / (ms * 1e-3)
+    return {
+        "workload": f"{count} urban members, sides from {{256,512,1024,2048}} (seed {seed}), "
+                    f"dx 2 m, {t_end:.0f} s each, per-member CFL dt, persistent graph-captured launches",
+        "members": count, "cells": int(sum(m.params.bed.numel() for m in members)),
+        "member_sides": {str(k): sides.count(k) for k in sorted(set(sides))},
+        "launches": launches, "member_steps_total": int(sum(i["steps_used"] for i in infos)),
+        "ms": ms, "wall_s": wall, "generation_s": gen_s,
+        "cell_updates_per_s": cups, "vs_plain_step": cups / plain_cups,
+        "rows": tune["rows"], "status": rc,
+        "max_abs_mass_balance_rel": max(abs(i["mass_balance_rel"]) for i in infos),
+    }
+
+
+def bench_adjoint(variant: str, n: int, steps: int, plain_cups: float):
+    """Config 5: checkpointed adjoint of a depth-RMSE loss at 10 output times."""
+    import numpy as np
+    import torch
+
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd import synthetic as sy
+    from hybrid_flood_amd.adjoint import RolloutModel, depth_rmse, rollout_depths
+
+    dev = torch.device("cuda", 0)
+    case = sy.urban_case(n, dx=2.0, seed=7)
+    h0 = sy.dam_break_state(n)
+    mask_t = torch.ones((n, n), dtype=torch.bool, device=dev)
+    h = torch.as_tensor(h0, device=dev)
+    state0 = nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h))
+    # The rollout horizon is the simulated time `steps` CFL steps reach.
+    p0 = nm.make_params(case.bed, case.manning_n, case.domain_mask, boundary_types=case.boundary_types,
+                        rainfall_times_s=case.rain_times_s, rainfall_rates_m_s=case.rain_rates_m_s,
+                        dx=case.dx, device=dev)
+    probe = nm.SWEState(*(x.clone() for x in state0))
+    horizon = nm.run_steps(probe, p0, steps, variant=variant, diagnostics=True)["final_time_s"]
+    times = np.linspace(0.0, horizon, 11).astype(np.float32)
+    model = RolloutModel(bed=p0.bed, domain_mask=mask_t, base_multiplier=torch.ones_like(h),
+                         h0=state0, output_times_s=times, boundary_types=case.boundary_types,
+                         rainfall_times_s=case.rain_times_s, rainfall_rates_m_s=case.rain_rates_m_s,
+                         dx=case.dx, max_steps=4 * steps, variant=variant)
+    hidden = torch.as_tensor(sy.smooth_field(n, np.random.default_rng(11), 0.01, 0.06), device=dev)
+    with torch.no_grad():
+        target = rollout_depths(hidden, torch.tensor(1.3), model).clone()
+    n0 = torch.as_tensor(case.manning_n, device=dev)
+    # plain forward of the same rollout
+    pl = model.params_for(n0, 1.0)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    res = nm.integrate_to_outputs(state0, pl, times, max_steps=4 * steps, variant=variant)
+    torch.cuda.synchronize(dev)
+    fwd_s = time.perf_counter() - t0
+    n_var = n0.clone().requires_grad_(True)
+    alpha = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    loss = depth_rmse(rollout_depths(n_var, alpha, model), target)
+    loss.backward()
+    torch.cuda.synchronize(dev)
+    grad_s = time.perf_counter() - t0
+    K = res.steps_used
+    cells = n * n
+    return {
+        "workload": f"urban {n}x{n}, dx 2 m, dam-break wet start, {K} CFL steps, loss = depth RMSE "
+                    "at 10 output times vs a run with a hidden smooth n-field and rain x1.3, "
+                    "sqrt(K) checkpoints",
+        "steps": K, "forward_s": fwd_s, "forward_plus_gradient_s": grad_s,
+        "gradient_vs_forward_time": grad_s / fwd_s,
+        "gradient_cell_steps_per_s": cells * K / grad_s,
+        "vs_plain_step": (cells * K / grad_s) / plain_cups,
+        "loss": float(loss), "grad_rain_multiplier": float(alpha.grad),
+        "grad_manning_l2": float(n_var.grad.norm()),
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +232,10 @@ def main(argv=None) -> int:
     ap.add_argument("--rows", type=int, default=0, help="rows per wavefront strip (0 = default)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for cpu_baseline (0 = skip)")
     ap.add_argument("--extra", action="store_true", help="also time the other kernel variant")
+    ap.add_argument("--ensemble", type=int, default=64, help="ensemble members (0 = skip)")
+    ap.add_argument("--ensemble-t", type=float, default=3600.0, help="simulated seconds per member")
+    ap.add_argument("--adjoint-n", type=int, default=1024, help="adjoint grid side (0 = skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
@@ -189,6 +310,10 @@ def main(argv=None) -> int:
         line["other_variant"] = {"variant": other, "steps": k,
                                  "cell_updates_per_s": cells * k / (ro["ms"] * 1e-3),
                                  "ms_per_step": ro["ms"] / k}
+    if args.ensemble > 0:
+        line["ensemble"] = bench_ensemble(args.variant, args.ensemble, 42, args.ensemble_t, value)
+    if args.adjoint_n > 0:
+        line["adjoint"] = bench_adjoint(args.variant, args.adjoint_n, args.adjoint_steps, value)
     if args.cpu_steps > 0:
         line["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -1,0 +1,135 @@
+"""Tile-shape autotuner: rows per wavefront strip of the fused step.
+
+A strip is 62 columns wide (one wavefront) and ``rows`` tall.  Taller strips
+amortise the halo row and the strip prologue; shorter strips give more
+wavefronts to hide memory latency and a smaller tail.  The best value depends
+on the grid shape and the kernel variant, so it is measured: each candidate
+runs a short device loop on a copy of the caller's state, timed with HIP
+events, and the winner is cached in a JSON file keyed by device
+architecture, grid shape and variant (``$SWE_TUNE_CACHE`` or
+``~/.cache/hybrid_flood_amd/tuning.json``).  Every candidate gives
+bit-identical states (the tiling does not change any cell's arithmetic).
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+
+import torch
+
+from . import _lib
+from .numerics import SWEParams, SWEState, context_for
+
+CANDIDATE_ROWS = (8, 12, 16, 24, 32, 48, 64)
+
+
+def cache_path() -> Path:
+    env = os.environ.get("SWE_TUNE_CACHE")
+    if env:
+        return Path(env)
+    return Path.home() / ".cache" / "hybrid_flood_amd" / "tuning.json"
+
+
+def _load_cache() -> dict:
+    try:
+        return json.loads(cache_path().read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(data: dict) -> None:
+    p = cache_path()
+    try:
+        p.parent.mkdir(parents=True, exist_ok=True)
+        tmp = p.with_suffix(".tmp")
+        tmp.write_text(json.dumps(data, indent=1, sort_keys=True))
+        tmp.replace(p)
+    except OSError:
+        pass  # a read-only home only loses the cache
+
+
+def _key(kind: str, shape, variant: str) -> str:
+    arch = torch.cuda.get_device_properties(torch.cuda.current_device()).gcnArchName.split(":")[0]
+    return f"{arch}/{kind}/{shape[0]}x{shape[1]}/{variant}"
+
+
+def time_loop_rows(ctx, state: SWEState, rows: int, steps: int) -> float:
+    """ms per step of `steps` device-loop steps with the given strip height."""
+    lib = _lib.load()
+    ctx.set_tile_rows(rows)
+    work = SWEState(*(x.clone() for x in state))
+    s = _lib.stream_handle(state.h.device)
+    st = _lib.state_struct(*work)
+    _lib.check(lib.swe_loop_begin(ctx.handle, st, 0.0, 2 * steps, s), "swe_loop_begin")
+    _lib.check(lib.swe_loop_advance(ctx.handle, steps, s), "swe_loop_advance")  # warm
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    _lib.check(lib.swe_loop_advance(ctx.handle, steps, s), "swe_loop_advance")
+    e1.record()
+    torch.cuda.synchronize(state.h.device)
+    _lib.check(lib.swe_loop_end(ctx.handle, st, None, s), "swe_loop_end")
+    return e0.elapsed_time(e1) / steps
+
+
+def tune_rows(params: SWEParams, state: SWEState, *, variant: str | None = None, steps: int = 32,
+              candidates=CANDIDATE_ROWS, use_cache: bool = True) -> dict:
+    """Pick and apply the fastest strip height for this grid; returns the record."""
+    from .numerics import DEFAULT_VARIANT
+
+    variant = variant or DEFAULT_VARIANT
+    ctx = context_for(params, variant)
+    key = _key("step", tuple(state.h.shape), variant)
+    cache = _load_cache() if use_cache else {}
+    rec = cache.get(key)
+    if rec is None:
+        chunk_steps = max(2, steps - steps % 2)
+        ctx.set_graph_chunk(chunk_steps)
+        timings = {int(r): time_loop_rows(ctx, state, int(r), chunk_steps) for r in candidates}
+        best = min(timings, key=timings.get)
+        rec = {"rows": best, "ms_per_step": timings[best],
+               "candidates": {str(k): v for k, v in timings.items()}}
+        if use_cache:
+            cache[key] = rec
+            _store_cache(cache)
+    ctx.set_tile_rows(int(rec["rows"]))
+    return rec
+
+
+def tune_ensemble_rows(runner, *, t_end: float = 1.0e9, launches: int = 16,
+                       candidates=(8, 16, 32), use_cache: bool = True) -> dict:
+    """Pick the strip height for an ensemble by timing whole ensemble launches."""
+    shapes = sorted(tuple(m.params.bed.shape) for m in runner.members)
+    key = _key("ensemble", (len(shapes), sum(a * b for a, b in shapes)), runner.variant)
+    cache = _load_cache() if use_cache else {}
+    rec = cache.get(key)
+    if rec is None:
+        timings = {}
+        saved = [SWEState(*(x.clone() for x in m.state)) for m in runner.members]
+        for r in candidates:
+            runner.set_tile_rows(int(r))
+            runner.set_graph_chunk(launches)
+            runner.begin(t_end, max_steps=2 * launches)
+            runner.advance(launches)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            runner.advance(launches)
+            e1.record()
+            torch.cuda.synchronize()
+            runner.end()
+            timings[int(r)] = e0.elapsed_time(e1) / launches
+            for m, s in zip(runner.members, saved):
+                for dst, src in zip(m.state, s):
+                    dst.copy_(src)
+        best = min(timings, key=timings.get)
+        rec = {"rows": best, "ms_per_launch": timings[best],
+               "candidates": {str(k): v for k, v in timings.items()}}
+        if use_cache:
+            cache[key] = rec
+            _store_cache(cache)
+    runner.set_tile_rows(int(rec["rows"]))
+    return rec
