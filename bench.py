@@ -130,25 +130,41 @@ def bench_ensemble(variant: str, count: int, seed: int, t_end: float, plain_cups
     tune = tune_ensemble_rows(runner)
     chunk = 32
     runner.set_graph_chunk(chunk)
+    total_cells = int(sum(m.params.bed.numel() for m in members))
     torch.cuda.synchronize(dev)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
+    evf = torch.cuda.Event(enable_timing=True)
     w0 = time.perf_counter()
     ev0.record()
     runner.begin(t_end, max_steps=200_000)
     launches = 0
+    full_launches = 0  # launches while every member was still running
+    last = count
+    full_open = True
     while True:
         runner.advance(chunk)
         launches += chunk
-        if runner.running() == 0:
+        running = runner.running()
+        if full_open and running == count:
+            full_launches = launches
+            evf.record()
+        else:
+            full_open = False
+        if running == 0:
             break
+        if running < last:
+            runner.compact()
+            last = running
     ev1.record()
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - w0
     rc, infos = runner.end()
     ms = ev0.elapsed_time(ev1)
+    ms_full = ev0.elapsed_time(evf) if full_launches else float("nan")
     cell_steps = sum(m.params.bed.numel() * i["steps_used"] for m, i in zip(members, infos))
     cups = cell_steps / (ms * 1e-3)
+    cups_full = total_cells * full_launches / (ms_full * 1e-3) if full_launches else None
     return {
         "workload": f"{count} urban members, sides from {{256,512,1024,2048}} (seed {seed}), "
                     f"dx 2 m, {t_end:.0f} s each, per-member CFL dt, persistent graph-captured launches",
@@ -157,6 +173,10 @@ def bench_ensemble(variant: str, count: int, seed: int, t_end: float, plain_cups
         "launches": launches, "member_steps_total": int(sum(i["steps_used"] for i in infos)),
         "ms": ms, "wall_s": wall, "generation_s": gen_s,
         "cell_updates_per_s": cups, "vs_plain_step": cups / plain_cups,
+        "all_members_running": {"launches": full_launches,
+                                "cell_updates_per_s": cups_full,
+                                "vs_plain_step": (cups_full / plain_cups) if cups_full else None},
+        "max_member_steps": int(max(i["steps_used"] for i in infos)),
         "rows": tune["rows"], "status": rc,
         "max_abs_mass_balance_rel": max(abs(i["mass_balance_rel"]) for i in infos),
     }

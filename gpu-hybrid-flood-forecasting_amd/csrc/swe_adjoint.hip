@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <string>
+#include <utility>
 
 #include "../../include/swe_hip.h"
 #include "swe_adjoint.hpp"
@@ -192,6 +193,13 @@ namespace {
 constexpr int kAdjRows = 16;
 }
 
+namespace {
+int launch_adjoint(SweContext* c, const float* h, const float* hu, const float* hv,
+                   const float* lh, const float* lhu, const float* lhv, float* oh, float* ohu,
+                   float* ohv, double dt, double time_s, float* grad_manning, float* grad_mult,
+                   hipStream_t s);
+}
+
 extern "C" int swe_adjoint_step(SweContext* c, SweState state, SweState lam_next, SweState lam_out,
                                 double dt, double time_s, float* grad_manning, float* grad_mult,
                                 void* stream) {
@@ -200,6 +208,49 @@ extern "C" int swe_adjoint_step(SweContext* c, SweState state, SweState lam_next
     return set_error(SWE_EINVAL, "swe_adjoint_step: null argument");
   if (lam_out.h == lam_next.h || lam_out.hu == lam_next.hu || lam_out.hv == lam_next.hv)
     return set_error(SWE_EINVAL, "swe_adjoint_step: lam_out must not alias lam_next");
+  return launch_adjoint(c, state.h, state.hu, state.hv, lam_next.h, lam_next.hu, lam_next.hv,
+                        lam_out.h, lam_out.hu, lam_out.hv, dt, time_s, grad_manning, grad_mult,
+                        (hipStream_t)stream);
+}
+
+extern "C" int swe_adjoint_sweep(SweContext* c, const float* states_h, const float* states_hu,
+                                 const float* states_hv, int32_t n, const double* dts,
+                                 const double* ts, SweState lam, SweState lam_tmp,
+                                 float* grad_manning, float* grad_mult, void* stream) {
+  if (!c || n < 0 || (n > 0 && (!states_h || !states_hu || !states_hv || !dts || !ts)) || !lam.h ||
+      !lam.hu || !lam.hv || !lam_tmp.h || !lam_tmp.hu || !lam_tmp.hv || lam.h == lam_tmp.h)
+    return set_error(SWE_EINVAL, "swe_adjoint_sweep: bad arguments");
+  const Grid& G = *context_grid(c);
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  hipStream_t s = (hipStream_t)stream;
+  // lambda ping-pongs between lam and lam_tmp; an odd count ends in lam_tmp,
+  // so copy back once at the end.
+  SweState a = lam, b = lam_tmp;
+  for (int k = n - 1; k >= 0; --k) {
+    int rc = launch_adjoint(c, states_h + (size_t)k * N, states_hu + (size_t)k * N,
+                            states_hv + (size_t)k * N, a.h, a.hu, a.hv, b.h, b.hu, b.hv, dts[k],
+                            ts[k], grad_manning, grad_mult, s);
+    if (rc) return rc;
+    std::swap(a, b);
+  }
+  if (a.h != lam.h) {
+    const size_t bytes = sizeof(float) * N;
+    if (hipMemcpyAsync(lam.h, a.h, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(lam.hu, a.hu, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(lam.hv, a.hv, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return set_error(SWE_EHIP, "swe_adjoint_sweep: copy back");
+  }
+  return SWE_OK;
+}
+
+namespace {
+int launch_adjoint(SweContext* c, const float* h, const float* hu, const float* hv,
+                   const float* lh, const float* lhu, const float* lhv, float* oh, float* ohu,
+                   float* ohv, double dt, double time_s, float* grad_manning, float* grad_mult,
+                   hipStream_t s) {
+  SweState state{const_cast<float*>(h), const_cast<float*>(hu), const_cast<float*>(hv)};
+  SweState lam_next{const_cast<float*>(lh), const_cast<float*>(lhu), const_cast<float*>(lhv)};
+  SweState lam_out{oh, ohu, ohv};
   const Grid& G = *context_grid(c);
   const double rate = context_rate(c, time_s);
   const float dt_f = (float)dt, dtrate_f = (float)(dt * rate);
@@ -207,7 +258,6 @@ extern "C" int swe_adjoint_step(SweContext* c, SweState state, SweState lam_next
   const int segs_x = (G.nx + kAdjCols - 1) / kAdjCols;
   const int n_waves = segs_x * ((G.ny + kAdjRows - 1) / kAdjRows);
   const int blocks = (n_waves + 3) / 4;
-  hipStream_t s = (hipStream_t)stream;
   if (context_variant(c) == SWE_VARIANT_REFERENCE)
     hipLaunchKernelGGL(k_adjoint_step<RefMath>, dim3(blocks), dim3(256), 0, s, G, state.h,
                        state.hu, state.hv, lam_next.h, lam_next.hu, lam_next.hv, lam_out.h,
@@ -222,3 +272,4 @@ extern "C" int swe_adjoint_step(SweContext* c, SweState state, SweState lam_next
   if (e != hipSuccess) return set_error(SWE_EHIP, std::string("swe_adjoint_step: ") + hipGetErrorString(e));
   return SWE_OK;
 }
+}  // namespace

@@ -27,6 +27,7 @@
 #include "swe_device.hpp"
 #include "swe_loop.hpp"
 #include "swe_strip.hpp"
+#include "swe_fast.hpp"
 #include "swe_host.hpp"
 
 namespace swe {
@@ -83,9 +84,15 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
     const float* src = mb.buf[par];
     float* dst = mb.buf[par ^ 1];
     StripAcc acc;
-    march_strip<M, true>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
-                         local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dt_f, ctl->dtrate_f,
-                         ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr, nullptr, nullptr, acc);
+    if constexpr (M::kFast)
+      march_strip_fast<true>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
+                             local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dt_f,
+                             ctl->dtrate_f, ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr,
+                             nullptr, nullptr, acc);
+    else
+      march_strip<M, true>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
+                           local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dt_f, ctl->dtrate_f,
+                           ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr, nullptr, nullptr, acc);
     const float inv = wave_max(acc.inv);
     const double vol = wave_sum(acc.vol);
     const double clip = wave_sum(acc.clip);
@@ -153,7 +160,9 @@ struct SweEnsemble {
   int rows = 16;
   int graph_chunk = 32;
   std::vector<SweMemberDesc> desc;
-  std::vector<Member> host;
+  std::vector<Member> host;  // every member, in the caller's order
+  std::vector<Member> act;   // the device table: running members, compacted
+  int act_rows = 16;
   std::vector<std::vector<double>> rain;  // per member [t | r]
   Member* d_mem = nullptr;
   LoopControl* d_ctl = nullptr;
@@ -174,18 +183,48 @@ struct SweEnsemble {
 
 namespace {
 
-int layout(SweEnsemble* e) {
+// Lay out the strips of the members flagged in `running` (all when null)
+// into the device member table.  When few cells remain, strips are made
+// shorter so that the launch still holds enough wavefronts to hide memory
+// latency (at least ~2 per resident slot), down to 4 rows.
+int layout(SweEnsemble* e, const std::vector<char>* running = nullptr) {
+  int dev = 0, cus = 0, per_cu = 0;
+  ENS_TRY(hipGetDevice(&dev));
+  ENS_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (e->variant == SWE_VARIANT_REFERENCE)
+    ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<RefMath>, 256, 0));
+  else
+    ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<FastMath>, 256, 0));
+  const long slots = (long)cus * std::max(per_cu, 1) * 4;  // resident wavefronts
+  auto waves_for = [&](int rows) {
+    long w = 0;
+    for (size_t m = 0; m < e->host.size(); ++m) {
+      if (running && !(*running)[m]) continue;
+      const Member& mb = e->host[m];
+      w += (long)((mb.G.nx + kStripCols - 1) / kStripCols) * ((mb.G.ny + rows - 1) / rows);
+    }
+    return w;
+  };
+  int rows = e->rows;
+  while (rows > 4 && waves_for(rows) < 2 * slots) rows = std::max(4, rows / 2);
+  e->act.clear();
   int base = 0;
-  for (auto& m : e->host) {
-    m.rows = e->rows;
-    m.segs_x = (m.G.nx + kStripCols - 1) / kStripCols;
-    m.n_waves = m.segs_x * ((m.G.ny + m.rows - 1) / m.rows);
-    m.wave_base = base;
-    base += m.n_waves;
+  for (size_t m = 0; m < e->host.size(); ++m) {
+    if (running && !(*running)[m]) continue;
+    Member mb = e->host[m];
+    mb.rows = rows;
+    mb.segs_x = (mb.G.nx + kStripCols - 1) / kStripCols;
+    mb.n_waves = mb.segs_x * ((mb.G.ny + rows - 1) / rows);
+    mb.wave_base = base;
+    base += mb.n_waves;
+    e->act.push_back(mb);
   }
+  e->act_rows = rows;
   e->total_waves = base;
   if (base > e->parts_cap) {
     if (e->P.vol) ENS_TRY(hipFree(e->P.vol));
+    e->P = EnsPartials{};
+    e->parts_cap = 0;
     const size_t bytes = (size_t)base * (3 * sizeof(double) + sizeof(float) + sizeof(int));
     char* p = nullptr;
     ENS_TRY(hipMalloc(&p, bytes));
@@ -196,14 +235,9 @@ int layout(SweEnsemble* e) {
     e->P.nf = reinterpret_cast<int*>(e->P.inv + base);
     e->parts_cap = base;
   }
-  int dev = 0, cus = 0, per_cu = 0;
-  ENS_TRY(hipGetDevice(&dev));
-  ENS_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  if (e->variant == SWE_VARIANT_REFERENCE)
-    ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<RefMath>, 256, 0));
-  else
-    ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<FastMath>, 256, 0));
-  e->grid = std::max(1, std::min(cus * std::max(per_cu, 1), (base + 3) / 4));
+  if (!e->act.empty())
+    ENS_TRY(hipMemcpy(e->d_mem, e->act.data(), sizeof(Member) * e->act.size(), hipMemcpyHostToDevice));
+  e->grid = std::max(1, (int)std::min<long>(slots / 4, ((long)base + 3) / 4));
   if (e->graph) {
     (void)hipGraphExecDestroy(e->graph);
     e->graph = nullptr;
@@ -214,10 +248,10 @@ int layout(SweEnsemble* e) {
 void launch_ens(SweEnsemble* e, hipStream_t s) {
   if (e->variant == SWE_VARIANT_REFERENCE)
     hipLaunchKernelGGL(k_ensemble_step<RefMath>, dim3(e->grid), dim3(256), 0, s, e->d_mem,
-                       (int)e->host.size(), e->total_waves, e->P);
+                       (int)e->act.size(), e->total_waves, e->P);
   else
     hipLaunchKernelGGL(k_ensemble_step<FastMath>, dim3(e->grid), dim3(256), 0, s, e->d_mem,
-                       (int)e->host.size(), e->total_waves, e->P);
+                       (int)e->act.size(), e->total_waves, e->P);
 }
 
 int launch_chunks(SweEnsemble* e, int launches, hipStream_t s) {
@@ -465,7 +499,8 @@ int swe_ensemble_begin(SweEnsemble* e, double t_end, int32_t max_steps, void* st
     e->host[m].L.max_steps = max_steps;
   }
   ENS_TRY(hipMemcpy(e->d_times, times.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
-  ENS_TRY(hipMemcpy(e->d_mem, e->host.data(), sizeof(Member) * n, hipMemcpyHostToDevice));
+  int lrc = layout(e);  // every member, in order: table index m == member m
+  if (lrc) return lrc;
   for (int m = 0; m < n; ++m) {
     const SweState& st = e->desc[m].state;
     const size_t N = (size_t)e->host[m].G.nx * e->host[m].G.ny;
@@ -501,6 +536,19 @@ int swe_ensemble_poll(SweEnsemble* e, int32_t* running, void* stream) {
   for (int m = 0; m < n; ++m) r += e->h_ctl[m].done ? 0 : 1;
   *running = r;
   return SWE_OK;
+}
+
+// Rebuild the launch over the members still running (host-synchronised: call
+// after swe_ensemble_poll, with no launch in flight).
+int swe_ensemble_compact(SweEnsemble* e, void* stream) {
+  if (!e || !e->active) return efail(SWE_EINVAL, "swe_ensemble_compact: no active run");
+  int32_t running = 0;
+  int rc = swe_ensemble_poll(e, &running, stream);
+  if (rc) return rc;
+  std::vector<char> flags(e->host.size());
+  for (size_t m = 0; m < flags.size(); ++m) flags[m] = e->h_ctl[m].done ? 0 : 1;
+  if (running == 0) return SWE_OK;
+  return layout(e, &flags);
 }
 
 int swe_ensemble_end(SweEnsemble* e, SweRunInfo* infos, void* stream) {
@@ -548,12 +596,17 @@ int swe_ensemble_run(SweEnsemble* e, double t_end, int32_t max_steps, SweRunInfo
   if (rc) return rc;
   const int chunk = e->graph_chunk > 0 ? e->graph_chunk : 16;
   long launched = 0;
+  int last_running = (int)e->host.size();
   while (true) {
     if ((rc = swe_ensemble_advance(e, chunk, stream))) return rc;
     launched += chunk;
     int running = 0;
     if ((rc = swe_ensemble_poll(e, &running, stream))) return rc;
     if (running == 0) break;
+    if (running < last_running) {  // drop finished members from the launch
+      if ((rc = swe_ensemble_compact(e, stream))) return rc;
+      last_running = running;
+    }
     if (launched > (long)max_steps + 2L * chunk)
       return efail(SWE_EHIP, "swe_ensemble_run: members did not stop at max_steps");
   }

@@ -27,6 +27,7 @@
 
 #include "swe_device.hpp"
 #include "swe_strip.hpp"
+#include "swe_fast.hpp"
 #include "swe_loop.hpp"
 #include "swe_host.hpp"
 #include "../../include/swe_hip.h"
@@ -56,9 +57,14 @@ __global__ __launch_bounds__(kBlock) void k_step(
   const int gw = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave);
   StripAcc acc;
   if (gw < n_waves) {
-    march_strip<M, MODE != MODE_STEP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x, rows,
-                                      dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi, cube_h,
-                                      cube_hu, cube_hv, acc);
+    if constexpr (M::kFast)
+      march_strip_fast<MODE != MODE_STEP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
+                                          rows, dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
+                                          cube_h, cube_hu, cube_hv, acc);
+    else
+      march_strip<M, MODE != MODE_STEP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
+                                        rows, dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
+                                        cube_h, cube_hu, cube_hv, acc);
   }
 
   if constexpr (MODE == MODE_LOOP_SPLIT) {
@@ -817,6 +823,61 @@ int swe_run_to_outputs(SweContext* c, SweState initial, const double* times, int
                   max_steps, k.out_idx, n_times, k.t);
     return fail(SWE_EMAXSTEPS, msg);
   }
+  return SWE_OK;
+}
+
+int swe_replay(SweContext* c, SweState in, const double* dts, const double* ts, int32_t n,
+               SweState out, void* stream) {
+  if (!c || !valid_state(in) || !valid_state(out) || (n > 0 && (!dts || !ts)) || n < 0)
+    return fail(SWE_EINVAL, "swe_replay: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  if ((rc = ensure_work(c))) return rc;
+  if (n == 0) return copy_state(c, out.h, out.hu, out.hv, in.h, in.hu, in.hv, s);
+  const LoopConst L = c->loop_const(0, 0);
+  const float* sh = in.h;
+  const float* shu = in.hu;
+  const float* shv = in.hv;
+  for (int k = 0; k < n; ++k) {
+    const int d = k & 1;  // work buffer written by step k (never the input of step k)
+    float* dh = c->buf(d, 0);
+    float* dhu = c->buf(d, 1);
+    float* dhv = c->buf(d, 2);
+    const double rate = host_interp(c, ts[k]);
+    launch_step_any(c, MODE_STEP, sh, shu, shv, dh, dhu, dhv, (float)dts[k],
+                    (float)(dts[k] * rate), (float)(dts[k] * c->gravity), L, nullptr, nullptr,
+                    nullptr, s);
+    sh = dh;
+    shu = dhu;
+    shv = dhv;
+  }
+  SWE_HIP_TRY(hipGetLastError());
+  return copy_state(c, out.h, out.hu, out.hv, sh, shu, shv, s);
+}
+
+int swe_replay_store(SweContext* c, SweState in, const double* dts, const double* ts, int32_t n,
+                     float* cube_h, float* cube_hu, float* cube_hv, void* stream) {
+  if (!c || !valid_state(in) || n < 0 || (n > 0 && (!dts || !ts || !cube_h || !cube_hu || !cube_hv)))
+    return fail(SWE_EINVAL, "swe_replay_store: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t N = c->cells();
+  const LoopConst L = c->loop_const(0, 0);
+  const float* sh = in.h;
+  const float* shu = in.hu;
+  const float* shv = in.hv;
+  for (int k = 0; k < n; ++k) {
+    float* dh = cube_h + (size_t)k * N;
+    float* dhu = cube_hu + (size_t)k * N;
+    float* dhv = cube_hv + (size_t)k * N;
+    const double rate = host_interp(c, ts[k]);
+    launch_step_any(c, MODE_STEP, sh, shu, shv, dh, dhu, dhv, (float)dts[k],
+                    (float)(dts[k] * rate), (float)(dts[k] * c->gravity), L, nullptr, nullptr,
+                    nullptr, s);
+    sh = dh;
+    shu = dhu;
+    shv = dhv;
+  }
+  SWE_HIP_TRY(hipGetLastError());
   return SWE_OK;
 }
 

@@ -94,6 +94,13 @@ SIGNATURES = {
     "swe_set_loop_finish": (c_int32, [c_void_p, c_int32]),
     "swe_adjoint_step": (c_int32, [c_void_p, SweState, SweState, SweState, c_double, c_double,
                                    c_void_p, c_void_p, c_void_p]),
+    "swe_adjoint_sweep": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                    POINTER(c_double), POINTER(c_double), SweState, SweState,
+                                    c_void_p, c_void_p, c_void_p]),
+    "swe_replay": (c_int32, [c_void_p, SweState, POINTER(c_double), POINTER(c_double), c_int32,
+                             SweState, c_void_p]),
+    "swe_replay_store": (c_int32, [c_void_p, SweState, POINTER(c_double), POINTER(c_double),
+                                   c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
     "swe_step": (c_int32, [c_void_p, SweState, SweState, c_double, c_double, c_void_p]),
     "swe_cfl_dt": (c_int32, [c_void_p, SweState, POINTER(c_double), c_void_p]),
     "swe_cfl_field": (c_int32, [c_void_p, SweState, c_void_p, c_void_p]),
@@ -120,6 +127,7 @@ SIGNATURES = {
     "swe_ensemble_begin": (c_int32, [c_void_p, c_double, c_int32, c_void_p]),
     "swe_ensemble_advance": (c_int32, [c_void_p, c_int32, c_void_p]),
     "swe_ensemble_poll": (c_int32, [c_void_p, POINTER(c_int32), c_void_p]),
+    "swe_ensemble_compact": (c_int32, [c_void_p, c_void_p]),
     "swe_ensemble_end": (c_int32, [c_void_p, POINTER(SweRunInfo), c_void_p]),
     "swe_ensemble_run": (c_int32, [c_void_p, c_double, c_int32, POINTER(SweRunInfo), c_void_p]),
     "swe_ensemble_total_waves": (c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),

@@ -79,11 +79,6 @@ def _clone(s: SWEState) -> SWEState:
     return SWEState(*(x.clone() for x in s))
 
 
-def _step(ctx, src: SWEState, dst: SWEState, dt: float, t: float) -> None:
-    _lib.check(_lib.load().swe_step(ctx.handle, _lib.state_struct(*src), _lib.state_struct(*dst),
-                                    float(dt), float(t), _lib.stream_handle(src.h.device)), "swe_step")
-
-
 def rollout_forward(params: SWEParams, state0: SWEState, times, *, max_steps: int,
                     variant: str, every: int = 0):
     """Device-loop forward; returns the depth stack (T, ny, nx) and the tape."""
@@ -116,18 +111,23 @@ def rollout_forward(params: SWEParams, state0: SWEState, times, *, max_steps: in
         while m < t64.shape[0] and clock[k] >= t64[m] - OUTPUT_SLACK_S:
             out_steps[m] = k
             m += 1
-    # Checkpoints by replay of the recorded schedule.
+    # Checkpoints by replay of the recorded schedule, one segment per call.
     C = every if every > 0 else max(1, int(math.ceil(math.sqrt(max(K, 1)))))
     cps = {0: _clone(state0)}
-    cur = _clone(state0)
-    nxt = SWEState(*(torch.empty_like(x) for x in state0))
-    for k in range(K):
-        _step(ctx, cur, nxt, dts[k], ts[k])
-        cur, nxt = nxt, cur
-        if (k + 1) % C == 0 and k + 1 < K:
-            cps[k + 1] = _clone(cur)
+    lib = _lib.load()
+    s = _lib.stream_handle(dev)
+    for start in range(0, K - C, C) if K > C else ():
+        nxt = SWEState(*(torch.empty_like(x) for x in state0))
+        _lib.check(lib.swe_replay(ctx.handle, _lib.state_struct(*cps[start]), _dptr(dts, start),
+                                  _dptr(ts, start), C, _lib.state_struct(*nxt), s), "swe_replay")
+        cps[start + C] = nxt
     tape = Tape(params, dts, ts, out_steps, cps, C, K)
     return res.states.h, res, tape
+
+
+def _dptr(arr: np.ndarray, offset: int):
+    """ctypes double* to arr[offset:] (arr must be a contiguous float64 array)."""
+    return ctypes.cast(arr.ctypes.data + 8 * int(offset), ctypes.POINTER(ctypes.c_double))
 
 
 def rollout_backward(tape: Tape, grad_depths: torch.Tensor, *, variant: str):
@@ -137,39 +137,49 @@ def rollout_backward(tape: Tape, grad_depths: torch.Tensor, *, variant: str):
     h0 = tape.checkpoints[0].h
     dev = h0.device
     s = _lib.stream_handle(dev)
+    N = h0.numel()
     gn = torch.zeros_like(h0)
     gm = torch.zeros_like(h0)
     lam = SWEState(torch.zeros_like(h0), torch.zeros_like(h0), torch.zeros_like(h0))
-    tmp = SWEState(torch.zeros_like(h0), torch.zeros_like(h0), torch.zeros_like(h0))
-    by_step: dict[int, list[int]] = {}
+    tmp = SWEState(torch.empty_like(h0), torch.empty_like(h0), torch.empty_like(h0))
+    outs_at: dict[int, list[int]] = {}
     for m, k in enumerate(tape.out_steps):
-        by_step.setdefault(int(k), []).append(m)
+        outs_at.setdefault(int(k), []).append(m)
     K = tape.steps
+    C = tape.every
+    # Segment cube: slot j holds the state entering step start + j.
+    cube = torch.empty((3, max(C, 1)) + tuple(h0.shape), dtype=h0.dtype, device=dev)
+    dts = np.ascontiguousarray(tape.dts, np.float64)
+    ts = np.ascontiguousarray(tape.ts, np.float64)
 
     def inject(k):
-        for m in by_step.get(k, ()):
+        for m in outs_at.get(k, ()):
             lam.h.add_(grad_depths[m])
 
     inject(K)
-    C = tape.every
-    seg_starts = sorted(tape.checkpoints)
-    for start in reversed(seg_starts):
-        stop = min(start + C, K)
-        if start >= stop:
+    pos = K  # lam is the cotangent of the state after `pos` steps
+    for start in sorted(tape.checkpoints, reverse=True):
+        if start >= pos:
             continue
-        states = [tape.checkpoints[start]]
-        for k in range(start, stop - 1):
-            nxt = SWEState(torch.empty_like(h0), torch.empty_like(h0), torch.empty_like(h0))
-            _step(ctx, states[-1], nxt, tape.dts[k], tape.ts[k])
-            states.append(nxt)
-        for k in range(stop - 1, start - 1, -1):
-            U = states[k - start]
-            _lib.check(lib.swe_adjoint_step(ctx.handle, _lib.state_struct(*U),
-                                            _lib.state_struct(*lam), _lib.state_struct(*tmp),
-                                            float(tape.dts[k]), float(tape.ts[k]), gn.data_ptr(),
-                                            gm.data_ptr(), s), "swe_adjoint_step")
-            lam, tmp = tmp, lam
-            inject(k)
+        n = pos - start
+        for f in range(3):
+            cube[f, 0].copy_(tape.checkpoints[start][f])
+        if n > 1:
+            _lib.check(lib.swe_replay_store(ctx.handle, _lib.state_struct(*tape.checkpoints[start]),
+                                            _dptr(dts, start), _dptr(ts, start), n - 1,
+                                            cube[0, 1].data_ptr(), cube[1, 1].data_ptr(),
+                                            cube[2, 1].data_ptr(), s), "swe_replay_store")
+        # reverse down to `start`, stopping at every output step to add its loss cotangent
+        stops = sorted((k for k in outs_at if start <= k < pos), reverse=True) + [start]
+        for k in stops:
+            if k < pos:
+                j = k - start
+                _lib.check(lib.swe_adjoint_sweep(
+                    ctx.handle, cube[0, j].data_ptr(), cube[1, j].data_ptr(), cube[2, j].data_ptr(),
+                    pos - k, _dptr(dts, k), _dptr(ts, k), _lib.state_struct(*lam),
+                    _lib.state_struct(*tmp), gn.data_ptr(), gm.data_ptr(), s), "swe_adjoint_sweep")
+                pos = k
+                inject(k)
     return gn, gm, lam
 
 

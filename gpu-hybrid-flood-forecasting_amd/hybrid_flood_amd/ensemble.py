@@ -114,6 +114,11 @@ class EnsembleRunner:
                    "swe_ensemble_poll")
         return r.value
 
+    def compact(self) -> None:
+        """Drop finished members from the launch (call between chunks)."""
+        _lib.check(_lib.load().swe_ensemble_compact(self.handle, _lib.stream_handle()),
+                   "swe_ensemble_compact")
+
     def end(self) -> tuple[int, list[dict]]:
         infos = self._infos()
         rc = _lib.load().swe_ensemble_end(self.handle, infos, _lib.stream_handle())

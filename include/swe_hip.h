@@ -201,6 +201,26 @@ int swe_adjoint_step(SweContext* ctx, SweState state, SweState lam_next, SweStat
                      double dt, double time_s, float* grad_manning, float* grad_mult,
                      void* stream);
 
+/* Replay a recorded schedule (dts[k], ts[k]: float64 step size and start
+ * time of step k, e.g. from swe_set_dt_log): `out` receives the state after
+ * n steps from `in` (they may alias).  Bit-identical to the device loop's
+ * steps.  swe_replay_store keeps every intermediate state: slot k of the
+ * (n, ny, nx) cubes is the state after step k.  Asynchronous. */
+int swe_replay(SweContext* ctx, SweState in, const double* dts, const double* ts, int32_t n,
+               SweState out, void* stream);
+int swe_replay_store(SweContext* ctx, SweState in, const double* dts, const double* ts, int32_t n,
+                     float* cube_h, float* cube_hu, float* cube_hv, void* stream);
+
+/* Reverse n consecutive steps: states_* hold the n states entering steps
+ * 0..n-1 of the segment as (n, ny, nx) cubes, dts/ts their schedule.  `lam`
+ * holds the cotangent of the state after the segment on entry and of the
+ * state entering it on return (lam_tmp is scratch of the same shape);
+ * sensitivities are added into grad_manning / grad_mult.  Asynchronous. */
+int swe_adjoint_sweep(SweContext* ctx, const float* states_h, const float* states_hu,
+                      const float* states_hv, int32_t n, const double* dts, const double* ts,
+                      SweState lam, SweState lam_tmp, float* grad_manning, float* grad_mult,
+                      void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched ensemble (no counterpart in the reference, which runs one domain;
  * it runs many independent rainfall/roughness scenarios of the same step).
@@ -229,6 +249,10 @@ int swe_ensemble_begin(SweEnsemble* ens, double t_end, int32_t max_steps, void* 
 int swe_ensemble_advance(SweEnsemble* ens, int32_t n_launches, void* stream);
 /* Synchronise and count members still running. */
 int swe_ensemble_poll(SweEnsemble* ens, int32_t* running, void* stream);
+/* Drop finished members from the launch (and shorten strips when few cells
+ * remain).  Synchronises; call between chunks.  swe_ensemble_run does this
+ * whenever a member finishes. */
+int swe_ensemble_compact(SweEnsemble* ens, void* stream);
 /* Copy final states out; fill infos[n_members]; SWE_EMAXSTEPS if a member
  * did not reach t_end, SWE_ENONFINITE if one went non-finite.  Synchronises. */
 int swe_ensemble_end(SweEnsemble* ens, SweRunInfo* infos, void* stream);

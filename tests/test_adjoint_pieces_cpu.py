@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import zlib
 
 import numpy as np
 import pytest
@@ -73,7 +74,7 @@ def _face_inputs(rng, regime):
 @pytest.mark.parametrize("xdir", [1, 0])
 @pytest.mark.parametrize("regime", ["sub", "supersonic_right", "supersonic_left", "wallA", "wallB", "step"])
 def test_face_vjp_matches_finite_differences(hc, xdir, regime):
-    rng = np.random.default_rng(hash((xdir, regime)) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(f"{xdir}{regime}".encode()))
     for trial in range(20):
         x = _face_inputs(rng, regime)
         gout = rng.normal(size=5)
@@ -118,7 +119,7 @@ def point_fwd(hc, x):
 
 @pytest.mark.parametrize("regime", ["wet", "clip", "shallow", "still"])
 def test_point_vjp_matches_finite_differences(hc, regime):
-    rng = np.random.default_rng(hash(regime) % 2**32)
+    rng = np.random.default_rng(zlib.crc32(regime.encode()))
     for _ in range(30):
         hs = rng.uniform(0.01, 0.5)
         hus, hvs = rng.normal(0, 0.2, 2)
@@ -145,7 +146,10 @@ def test_point_vjp_matches_finite_differences(hc, regime):
             bp[k] += eps
             bm[k] -= eps
             fd.append((f(*bp) - f(*bm)) / (2 * eps))
-        np.testing.assert_allclose(out[:3], fd[:3], rtol=1e-5, atol=1e-8)
+        # still water sits on the |q| kink: the central difference there is off by
+        # O(K/q * eps); compare it with a looser bar
+        rtol = 1e-3 if regime == "still" else 1e-5
+        np.testing.assert_allclose(out[:3], fd[:3], rtol=rtol, atol=1e-8)
         np.testing.assert_allclose(out[3], fd[3], rtol=1e-5, atol=1e-8)
         np.testing.assert_allclose(out[4], out[0] * dtrate, rtol=1e-15)
 
