@@ -242,6 +242,23 @@ def bench_adjoint(variant: str, n: int, steps: int, plain_cups: float):
     }
 
 
+def copy_calibration() -> int:
+    """Four device-to-device copies of 256 MiB each: 256 MiB read and 256 MiB
+    written per dispatch, the reference for the per-dispatch HBM byte counters."""
+    import torch
+
+    n = 256 * 1024 * 1024 // 4
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(4):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"dispatches": 4, "bytes_read_per_dispatch": n * 4,
+                                           "bytes_written_per_dispatch": n * 4}}), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,7 +273,12 @@ def main(argv=None) -> int:
     ap.add_argument("--ensemble-t", type=float, default=3600.0, help="simulated seconds per member")
     ap.add_argument("--adjoint-n", type=int, default=1024, help="adjoint grid side (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 4 device copies of 256 MiB (the known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
               "(the step is measured on one MI355X; multi-GPU slab sharding is design only).",

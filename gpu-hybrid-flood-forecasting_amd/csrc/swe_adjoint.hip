@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
     const float* __restrict__ LHV, float* __restrict__ OH, float* __restrict__ OHU,
     float* __restrict__ OHV, float* __restrict__ grad_n, float* __restrict__ grad_mult, float dt,
     float dtrate, float dtg, int rows, int segs_x, int n_waves) {
-  const int gw = blockIdx.x * (blockDim.x / kWave) + (int)(threadIdx.x / kWave);
+  const int gw = blockIdx.x * (blockDim.x / kWave) + wave_in_block();
   if (gw >= n_waves) return;
   const int lane = threadIdx.x & (kWave - 1);
   const int sx = gw % segs_x, sy = gw / segs_x;

@@ -249,18 +249,35 @@ __device__ __forceinline__ float cfl_inverse(const Grid& G, float h, float hu, f
 // ---------------------------------------------------------------------------
 // Cross-lane moves inside a wavefront (gfx9 DPP wave shifts)
 // ---------------------------------------------------------------------------
-// Lane l receives lane l-1's value (lane 0 receives 0).
-__device__ __forceinline__ float from_west(float x) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xF, 0xF, false));
-}
+// Lane l receives lane l-1's value (lane 0 receives 0).  bound_ctrl makes
+// the lanes without a source read zero, so no old value has to be set up.
 __device__ __forceinline__ int from_west(int x) {
-  return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(x, 0x138, 0xF, 0xF, true);
+}
+__device__ __forceinline__ float from_west(float x) {
+  return __builtin_bit_cast(float, from_west(__builtin_bit_cast(int, x)));
 }
 // Lane l receives lane l+1's value (lane 63 receives 0).
 __device__ __forceinline__ float from_east(float x) {
   return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xF, 0xF, false));
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x130, 0xF, 0xF, true));
+}
+
+// Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
+// Renumbering them so that every XCD sweeps one contiguous band of the
+// launch keeps neighbouring strips, which share halo rows and the cache
+// lines at their edges, in one L2.  A bijection on [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int per = nb / kXcd, rem = nb % kXcd;
+  const int x = b % kXcd, k = b / kXcd;
+  return x * per + min(x, rem) + k;
+}
+
+// Index of the calling wavefront within its workgroup, as a wave-uniform
+// (scalar) value so that everything derived from it stays in SGPRs.
+__device__ __forceinline__ int wave_in_block() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
 }
 
 // Wavefront reductions (all lanes receive the result).

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
                                                       EnsPartials P) {
   const int lane = threadIdx.x & (kWave - 1);
   const int stride = gridDim.x * (blockDim.x / kWave);
-  for (int w = blockIdx.x * (blockDim.x / kWave) + (int)(threadIdx.x / kWave); w < total_waves;
+  for (int w = blockIdx.x * (blockDim.x / kWave) + wave_in_block(); w < total_waves;
        w += stride) {
     const int m = find_member(mem, n_members, w);
     const Member& mb = mem[m];

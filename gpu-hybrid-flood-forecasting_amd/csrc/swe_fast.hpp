@@ -11,8 +11,8 @@
 //     zero mass: a*q + b*(-q) with a == b;
 //   - one reciprocal of max(h, dry) shared by the friction law and the CFL
 //     epilogue; x^(7/3) as x^2 * 2^(log2(x)/3 * ...) on the hardware log/exp;
-//   - depth and clip volume summed per lane in float32 over the strip's rows,
-//     then in float64 across lanes and workgroups.
+//   - depth and clip volume summed in float64 per cell, so the totals do not
+//     depend on the strip shape.
 // Results stay within the stated tolerance of the FP64 oracle; the lake at
 // rest, wall mass flux and open-boundary accounting keep their exact forms.
 #pragma once
@@ -124,27 +124,95 @@ __device__ __forceinline__ FaceFlux face_fast_any(const Grid& G, const SideF& A,
                                B.h2, B.a);
 }
 
+// Raw inputs of one lane's cell in one row, as loaded (nothing is applied to
+// them until they are used, so the loads stay in flight).
+struct RawCell {
+  float h, hu, hv, b;
+  unsigned a;
+  bool neg_hv;  // row beyond a reflective south/north edge
+};
+
+// Per-strip addressing through raw buffer resources.  Each array gets a
+// resource based at the strip's lowest row; a row is selected by a
+// wave-uniform byte offset (scalar register) and each lane adds its fixed
+// column offset, so no vector address arithmetic runs per row.  Loads past a
+// resource's end read 0 and stores past it are dropped.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, size_t bytes) {
+  const int records = bytes > 0xFFFFFFFFull ? -1 : (int)(unsigned)bytes;  // uint32 field
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, records,
+                                           0x00020000);
+}
+
+struct StripIO {
+  Rsrc h, hu, hv, bed, n, mult, mask;
+  unsigned off4, off1;  // this lane's column, in bytes, in a float / uint8 row
+  unsigned pitch;       // bytes per float row
+  int jlo;              // row the resources start at
+
+  __device__ __forceinline__ unsigned row4(int jj) const { return (unsigned)(jj - jlo) * pitch; }
+  __device__ __forceinline__ float ld(Rsrc r, int jj) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off4, row4(jj), 0));
+  }
+  __device__ __forceinline__ unsigned ld_mask(int jj) const {
+    return __builtin_amdgcn_raw_buffer_load_b8(mask, off1, (unsigned)(jj - jlo) * (pitch >> 2), 0);
+  }
+};
+
+__device__ __forceinline__ StripIO strip_io(const Grid& G, const float* H, const float* HU,
+                                            const float* HV, int jlo, int ic) {
+  StripIO io;
+  const size_t row0 = (size_t)jlo * (size_t)G.nx;
+  const size_t cells = (size_t)(G.ny - jlo) * (size_t)G.nx;
+  io.h = make_rsrc(H + row0, cells * 4);
+  io.hu = make_rsrc(HU + row0, cells * 4);
+  io.hv = make_rsrc(HV + row0, cells * 4);
+  io.bed = make_rsrc(G.bed + row0, cells * 4);
+  io.n = make_rsrc(G.n + row0, cells * 4);
+  io.mult = make_rsrc(G.mult + row0, cells * 4);
+  io.mask = make_rsrc(G.mask + row0, cells);
+  io.off4 = (unsigned)ic * 4u;
+  io.off1 = (unsigned)ic;
+  io.pitch = (unsigned)G.nx * 4u;
+  io.jlo = jlo;
+  return io;
+}
+
+// Row j of the lane's column: clamped row index (numerics.py:29-55); the
+// reflective-edge sign flips are applied in side_of.
+__device__ __forceinline__ RawCell load_raw(const Grid& G, const StripIO& io, int j) {
+  const int jj = min(max(j, 0), G.ny - 1);
+  RawCell r;
+  r.h = io.ld(io.h, jj);
+  r.hu = io.ld(io.hu, jj);
+  r.hv = io.ld(io.hv, jj);
+  r.b = io.ld(io.bed, jj);
+  r.a = io.ld_mask(jj);
+  r.neg_hv = (j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0);
+  return r;
+}
+
 struct RowCell {
   SideF s;
   float hu, hv;
 };
 
-template <class M>
-__device__ __forceinline__ RowCell load_fast(const ColumnReader<M>& rd, const Grid& G, int j) {
-  const int jj = min(max(j, 0), rd.ny - 1);
-  const size_t idx = (size_t)jj * (size_t)rd.nx + (size_t)rd.ic;
-  float h = rd.H[idx], hu = rd.HU[idx], hv = rd.HV[idx];
-  const float b = G.bed[idx];
-  const int a = G.mask[idx] != 0;
-  if (rd.neg_hu) hu = -hu;
-  if ((j < 0 && rd.refl_south) || (j >= rd.ny && rd.refl_north)) hv = -hv;
+// Cell quantities of a loaded row, with the normal momentum negated beyond a
+// reflective edge (numerics.py:29-55).
+__device__ __forceinline__ RowCell side_of(const RawCell& r, bool neg_hu, float dry) {
   RowCell c;
-  c.s = make_side(h, hu, hv, b, a, G.dry);
+  const float hu = neg_hu ? -r.hu : r.hu;
+  const float hv = r.neg_hv ? -r.hv : r.hv;
+  c.s = make_side(r.h, hu, hv, r.b, r.a != 0, dry);
   c.hu = hu;
   c.hv = hv;
   return c;
 }
 
+// One wavefront's strip: columns sx*62-1 .. sx*62+62, rows sy*rows ..
+// sy*rows+rows-1.  sx, sy and rows must be wave-uniform.  The raw inputs of
+// row j+2 are in flight while row j is computed.
 template <bool LOOP>
 __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
@@ -160,32 +228,47 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
   const int lane = threadIdx.x & (kWave - 1);
   const int i = sx * kStripCols - 1 + lane;
   const bool own = lane >= 1 && lane <= kStripCols && i < G.nx;
-  ColumnReader<FastMath> rd;
-  rd.H = H;
-  rd.HU = HU;
-  rd.HV = HV;
-  rd.ic = min(max(i, 0), G.nx - 1);
-  rd.neg_hu = (i < 0 && G.bt[0] == 0) || (i >= G.nx && G.bt[1] == 0);
-  rd.nx = G.nx;
-  rd.ny = G.ny;
-  rd.refl_south = G.bt[2] == 0;
-  rd.refl_north = G.bt[3] == 0;
+  const int ic = min(max(i, 0), G.nx - 1);
+  const bool neg_hu = (i < 0 && G.bt[0] == 0) || (i >= G.nx && G.bt[1] == 0);
+  // Only the first and last strip of a row band touch the west/east edge faces.
+  const bool edge_x = LOOP && (sx == 0 || (sx + 1) * kStripCols >= G.nx);
   const int j0 = sy * rows;
   const int j1 = min(j0 + rows, G.ny);
   const float rdx = G.rdx, rdy = G.rdy, dry = G.dry;
+  const StripIO io = strip_io(G, H, HU, HV, max(j0 - 1, 0), ic);
+  const size_t out0 = (size_t)j0 * (size_t)G.nx, out_cells = (size_t)(j1 - j0) * (size_t)G.nx;
+  const Rsrc oh = make_rsrc(Ho + out0, out_cells * 4);
+  const Rsrc ohu = make_rsrc(HUo + out0, out_cells * 4);
+  const Rsrc ohv = make_rsrc(HVo + out0, out_cells * 4);
 
-  const RowCell below = load_fast(rd, G, j0 - 1);
-  RowCell cur = load_fast(rd, G, j0);
-  FaceFlux fy = face_fast_any<false>(G, below.s, cur.s, __all(below.s.a && cur.s.a));
-  float yt0 = fy.f0, yt1 = fy.ft, yt2 = fy.fn + fy.cB;
-  if (LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
-  float vol32 = 0.0f, clip32 = 0.0f;
+  // Prologue: the south face of row j0, rows j0 (current) and j0+1 (in flight).
+  RowCell cur = side_of(load_raw(G, io, j0 - 1), neg_hu, dry);
+  RawCell next = load_raw(G, io, j0);
+  float yt0, yt1, yt2;
+  {
+    const RowCell c0 = side_of(next, neg_hu, dry);
+    next = load_raw(G, io, j0 + 1);
+    const FaceFlux fy = face_fast_any<false>(G, cur.s, c0.s, __all(cur.s.a && c0.s.a));
+    if (LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
+    yt0 = fy.f0;
+    yt1 = fy.ft;
+    yt2 = fy.fn + fy.cB;
+    cur = c0;
+  }
+  float nn = io.ld(io.n, j0);
+  float mult = io.ld(io.mult, j0);
+  float nf_probe = 0.0f;
 
-  for (int j = j0; j < j1; ++j) {
-    const RowCell up = load_fast(rd, G, j + 1);
-    const size_t idx = (size_t)j * (size_t)G.nx + (size_t)rd.ic;
-    const float nn = G.n[idx];
-    const float mult = G.mult[idx];
+  // Row j: `cur` is row j, `north` row j+1 (loaded one row ago).  Issues the
+  // loads of row j+2 into `ahead` and of row j+1's n/multiplier into
+  // nn_next/mult_next, and leaves row j+1's side in `up`.
+  auto row = [&](int j, const RowCell& cur, RowCell& up, const RawCell& north, RawCell& ahead,
+                 float nn_j, float mult_j, float& nn_next, float& mult_next) {
+    // Unconditional, so the wait counts stay exact; past the strip the
+    // clamped rows were just loaded and come from the cache.
+    ahead = load_raw(G, io, min(j + 2, j1));
+    nn_next = io.ld(io.n, min(j + 1, j1 - 1));
+    mult_next = io.ld(io.mult, min(j + 1, j1 - 1));
 
     SideF w;
     w.h = from_west(cur.s.h);
@@ -200,18 +283,18 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
     const float xl0 = from_east(fx.f0);
     const float xl1 = from_east(fx.fn + fx.cA);
     const float xl2 = from_east(fx.ft);
-    if (LOOP) {
+    if (edge_x) {
       if (i == 0) acc.bnd += (double)fx.f0 * dyd;
       if (i == G.nx) acc.bnd -= (double)fx.f0 * dyd;
     }
+    up = side_of(north, neg_hu, dry);
     const FaceFlux gy = face_fast_any<false>(G, cur.s, up.s, __all(cur.s.a && up.s.a));
     const float yb0 = gy.f0, yb1 = gy.ft, yb2 = gy.fn + gy.cA;
-    if (LOOP && j + 1 == G.ny && own) acc.bnd -= (double)gy.f0 * dxd;
 
     const float d0 = fmaf(yb0 - yt0, rdy, (xl0 - xr0) * rdx);
     const float d1 = fmaf(yb1 - yt1, rdy, (xl1 - xr1) * rdx);
     const float d2 = fmaf(yb2 - yt2, rdy, (xl2 - xr2) * rdx);
-    const float hs = fmaf(dtrate, mult, fmaf(-dt, d0, cur.s.h));
+    const float hs = fmaf(dtrate, mult_j, fmaf(-dt, d0, cur.s.h));
     const float hus = fmaf(-dt, d1, cur.hu);
     const float hvs = fmaf(-dt, d2, cur.hv);
     const float hn = fmaxf(hs, 0.0f);
@@ -220,17 +303,21 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
     const float q = __builtin_amdgcn_sqrtf(fmaf(hus, hus, hvs * hvs));
     // 1 / hd^(7/3) = (1/hd)^2 * 2^(-log2(hd) * (7/3 - 2)); K = dtg n^2 q / hd^(7/3)
     const float ip13 = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(hd) * -kPow73MinusTwo);
-    const float K = ((dtg * (nn * nn)) * q) * ((rhd * rhd) * ip13);
+    const float K = ((dtg * (nn_j * nn_j)) * q) * ((rhd * rhd) * ip13);
     const float rD = __builtin_amdgcn_rcpf(1.0f + K);
     const bool wet = cur.s.a && (hn > dry);
     const float h_o = cur.s.a ? hn : 0.0f;
     const float hu_o = wet ? hus * rD : 0.0f;
     const float hv_o = wet ? hvs * rD : 0.0f;
     if (own) {
-      Ho[idx] = h_o;
-      HUo[idx] = hu_o;
-      HVo[idx] = hv_o;
+      const unsigned so = (unsigned)(j - j0) * io.pitch;
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, h_o), oh, io.off4, so, 0);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, hu_o), ohu, io.off4, so,
+                                            0);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, hv_o), ohv, io.off4, so,
+                                            0);
       if (LOOP) {
+        const size_t idx = (size_t)j * (size_t)G.nx + (size_t)ic;
         for (int k = rec_lo; k < rec_hi; ++k) {
           cube_h[(size_t)k * N + idx] = h_o;
           cube_hu[(size_t)k * N + idx] = hu_o;
@@ -241,20 +328,32 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
         const float ws = __builtin_amdgcn_sqrtf(G.g * h_o);
         const float inv = fmaf(fabsf(hv_o * rhd) + ws, rdy, (fabsf(hu_o * rhd) + ws) * rdx);
         acc.inv = fmaxf(acc.inv, cur.s.a ? inv : 0.0f);
-        vol32 += h_o;
-        clip32 += cur.s.a ? fmaxf(-hs, 0.0f) : 0.0f;
-        acc.nf |= !(isfinite(h_o) && isfinite(hu_o) && isfinite(hv_o));
+        acc.vol += (double)h_o;
+        const float clip = cur.s.a ? fmaxf(-hs, 0.0f) : 0.0f;
+        acc.clip += (double)clip;
+        // (x + y + z) * 0 stays 0 unless one of them is infinite or NaN
+        nf_probe = fmaf(h_o + hu_o + hv_o, 0.0f, nf_probe);
       }
     }
     yt0 = gy.f0;
     yt1 = gy.ft;
     yt2 = gy.fn + gy.cB;
-    cur = up;
+  };
+
+  // Two rows per trip with the carried values alternating between two sets
+  // of registers, so nothing is copied at the loop edge.
+  RowCell alt;
+  RawCell next2;
+  float nn2 = 0.0f, mult2 = 0.0f;
+  int j = j0;
+  for (; j + 1 < j1; j += 2) {
+    row(j, cur, alt, next, next2, nn, mult, nn2, mult2);
+    row(j + 1, alt, cur, next2, next, nn2, mult2, nn, mult);
   }
-  if (LOOP) {
-    acc.vol += (double)vol32;
-    acc.clip += (double)clip32;
-  }
+  if (j < j1) row(j, cur, alt, next, next2, nn, mult, nn2, mult2);
+  // yt0 now holds the mass flux through the north face of row j1-1.
+  if (LOOP && j1 == G.ny && own) acc.bnd -= (double)yt0 * dxd;
+  if (LOOP) acc.nf |= !isfinite(nf_probe);
 }
 
 }  // namespace swe

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
     rec_lo = ctl->rec_lo;
     rec_hi = ctl->rec_hi;
   }
-  const int gw = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave);
+  const int gw = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave_in_block();
   StripAcc acc;
   if (gw < n_waves) {
     if constexpr (M::kFast)
