@@ -56,8 +56,15 @@ def time_loop(params, h0, steps, warmup, variant, rows=None):
     dev = params.bed.device
     lib = _lib.load()
     ctx = nm.context_for(params, variant)
+    tuned = None
     if rows:
         ctx.set_tile_rows(rows)
+    else:
+        from hybrid_flood_amd.autotune import tune_rows
+
+        h = torch.as_tensor(h0, device=dev).contiguous()
+        tuned = tune_rows(params, nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h)),
+                          variant=variant)
     chunk = _even_divisor(steps)
     ctx.set_graph_chunk(chunk)
     warm = max(chunk, int(math.ceil(warmup / chunk)) * chunk) if warmup > 0 else 0
@@ -82,7 +89,9 @@ def time_loop(params, h0, steps, warmup, variant, rows=None):
     _lib.check(lib.swe_loop_end(ctx.handle, st, ctypes.byref(info), s), "swe_loop_end")
     if info.steps_used != warm + steps:
         raise RuntimeError(f"device loop took {info.steps_used} steps, expected {warm + steps}")
-    return {"ms": ms, "wall_s": wall, "warmup": warm, "info": info.as_dict(), "chunk": chunk}
+    return {"ms": ms, "wall_s": wall, "warmup": warm, "info": info.as_dict(), "chunk": chunk,
+            "tile": ({"rows": tuned["rows"], "waves_per_cu": tuned["waves_per_cu"], "autotuned": True}
+                     if tuned else {"rows": rows, "waves_per_cu": 0, "autotuned": False})}
 
 
 def cpu_baseline(case, steps):
@@ -325,6 +334,7 @@ def main(argv=None) -> int:
             "cells": cells,
             "kernel_variant": args.variant,
             "graph_chunk": r["chunk"],
+            "tile": r["tile"],
         },
         "roofline": {
             "bound": "hbm",

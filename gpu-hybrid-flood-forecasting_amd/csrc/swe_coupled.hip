@@ -1,0 +1,246 @@
+// Kernels of the hybrid coupled forecast around the step
+// (hybrid/coupled_forecast.py:53-120):
+//   - network_features: the six normalised CNN input channels of a state in
+//     one pass (depth, the two safe velocities, the rain channel, terrain,
+//     roughness), written in any channel/cell stride (NHWC or NCHW);
+//   - apply_residual_correction, fused with the residual clip of
+//     coupled_forecast.py:199-201: positivity clip, a global rescale that
+//     restores the physics depth volume, and the velocity correction.  The
+//     two volume sums are float64 (per-workgroup partials in a fixed order,
+//     then one workgroup), so the rescale is deterministic and the corrected
+//     volume matches the physics volume to float32 rounding of each cell.
+// Element-wise arithmetic follows the reference's float32 order (the build
+// uses -ffp-contract=off and IEEE division).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <string>
+
+#include "../../include/swe_hip.h"
+#include "swe_device.hpp"
+#include "swe_host.hpp"
+#include "swe_loop.hpp"
+
+namespace swe {
+namespace {
+
+// coupled_forecast.py:49-50: h > dry ? q / max(h, dry) : 0
+__device__ __forceinline__ float safe_velocity(float q, float h, float dry) {
+  return h > dry ? q / fmaxf(h, dry) : 0.0f;
+}
+
+// jnp.clip(x, -lim, lim), NaN passes through.
+__device__ __forceinline__ float clip_to(float x, float lim) {
+  return x < -lim ? -lim : (x > lim ? lim : x);
+}
+
+struct Norm {
+  float mean[3], std[3];
+};
+
+__global__ __launch_bounds__(kBlock) void k_features(Grid G, const float* __restrict__ H,
+                                                     const float* __restrict__ HU,
+                                                     const float* __restrict__ HV,
+                                                     const float* __restrict__ terrain,
+                                                     const float* __restrict__ rough, Norm nz,
+                                                     float rain_channel, float* __restrict__ out,
+                                                     long long cs, long long ps) {
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    const float h = H[k];
+    const float u = safe_velocity(HU[k], h, G.dry);
+    const float v = safe_velocity(HV[k], h, G.dry);
+    const bool a = G.mask[k] != 0;
+    float* o = out + (long long)k * ps;
+    o[0] = a ? (h - nz.mean[0]) / nz.std[0] : 0.0f;
+    o[cs] = a ? (u - nz.mean[1]) / nz.std[1] : 0.0f;
+    o[2 * cs] = a ? (v - nz.mean[2]) / nz.std[2] : 0.0f;
+    o[3 * cs] = rain_channel;
+    o[4 * cs] = terrain[k];
+    o[5 * cs] = rough[k];
+  }
+}
+
+struct Residual {
+  const float* r;
+  long long cs, ps;
+  float relax;
+  float lim[3];  // +inf: no clip
+  __device__ __forceinline__ float at(size_t k, int c) const {
+    return clip_to(r[(long long)k * ps + c * cs], lim[c]);
+  }
+};
+
+// Pass 1: per-workgroup float64 sums of the masked physics depth and of the
+// clipped provisional depth (coupled_forecast.py:97-105).
+__global__ __launch_bounds__(kBlock) void k_corr_sums(Grid G, const float* __restrict__ H,
+                                                      Residual R, double* __restrict__ part) {
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  double tgt = 0.0, prov = 0.0;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    const float h = H[k];
+    const bool a = G.mask[k] != 0;
+    const float p = a ? fmaxf(h + R.relax * R.at(k, 0), 0.0f) : 0.0f;
+    tgt += a ? (double)h : 0.0;
+    prov += (double)p;
+  }
+  __shared__ double s_t[kWavesPerBlock], s_p[kWavesPerBlock];
+  tgt = wave_sum(tgt);
+  prov = wave_sum(prov);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_t[w] = tgt;
+    s_p[w] = prov;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+      a += s_t[i];
+      b += s_p[i];
+    }
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// Pass 2 (one workgroup): totals and the rescale factor; out[0..3] =
+// (target volume, provisional volume, scale, use_scaled) in cell units.
+__global__ __launch_bounds__(kBlock) void k_corr_scale(const double* __restrict__ part, int nparts,
+                                                       double* __restrict__ out) {
+  double tgt = 0.0, prov = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kBlock) {
+    tgt += part[2 * i];
+    prov += part[2 * i + 1];
+  }
+  __shared__ double s_t[kWavesPerBlock], s_p[kWavesPerBlock];
+  tgt = wave_sum(tgt);
+  prov = wave_sum(prov);
+  const int w = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s_t[w] = tgt;
+    s_p[w] = prov;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+      a += s_t[i];
+      b += s_p[i];
+    }
+    const double eps = (double)FLT_EPSILON;  // jnp.finfo(float32).eps
+    out[0] = a;
+    out[1] = b;
+    out[2] = b > eps ? a / b : 1.0;
+    out[3] = a > eps ? 1.0 : 0.0;
+  }
+}
+
+// Pass 3: corrected state (coupled_forecast.py:106-120).
+// The corrected state may be the predicted one (each cell is read before it
+// is written, by the same thread), so no pointer here is __restrict__.
+__global__ __launch_bounds__(kBlock) void k_corr_apply(Grid G, const float* H, const float* HU,
+                                                       const float* HV, Residual R,
+                                                       const double* sc, float* Ho, float* HUo,
+                                                       float* HVo) {
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  const double scale = sc[2];
+  const bool use_scaled = sc[3] != 0.0;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    const float h = H[k];
+    const bool a = G.mask[k] != 0;
+    const float u = safe_velocity(HU[k], h, G.dry);
+    const float v = safe_velocity(HV[k], h, G.dry);
+    const float p = a ? fmaxf(h + R.relax * R.at(k, 0), 0.0f) : 0.0f;
+    const float ch = use_scaled ? (float)((double)p * scale) : h;
+    const float cu = u + R.relax * R.at(k, 1);
+    const float cv = v + R.relax * R.at(k, 2);
+    const bool wet = a && ch > G.dry;
+    Ho[k] = ch;
+    HUo[k] = wet ? ch * cu : 0.0f;
+    HVo[k] = wet ? ch * cv : 0.0f;
+  }
+}
+
+int grid_blocks(size_t N) {
+  return (int)std::max<size_t>(1, std::min<size_t>((N + kBlock - 1) / kBlock, 4096));
+}
+
+bool valid(const SweState& s) { return s.h && s.hu && s.hv; }
+
+}  // namespace
+}  // namespace swe
+
+using namespace swe;
+
+extern "C" {
+
+int swe_network_features(SweContext* c, SweState state, const float* terrain_std,
+                         const float* roughness_std, const float* state_mean,
+                         const float* state_std, double rainfall_scale_mm_hr, double time_s,
+                         float* features, int64_t channel_stride, int64_t cell_stride,
+                         void* stream) {
+  if (!c || !valid(state) || !terrain_std || !roughness_std || !state_mean || !state_std ||
+      !features)
+    return set_error(SWE_EINVAL, "swe_network_features: null argument");
+  const Grid& G = *context_grid(c);
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  Norm nz;
+  for (int i = 0; i < 3; ++i) {
+    nz.mean[i] = state_mean[i];
+    nz.std[i] = state_std[i];
+  }
+  // coupled_forecast.py:65-69 in float32: rate * 3.6e6, then / scale.
+  const float mm_hr = (float)context_rate(c, time_s) * 3.6e6f;
+  const float rain_channel = mm_hr / (float)rainfall_scale_mm_hr;
+  hipLaunchKernelGGL(k_features, dim3(grid_blocks(N)), dim3(kBlock), 0, (hipStream_t)stream, G,
+                     state.h, state.hu, state.hv, terrain_std, roughness_std, nz, rain_channel,
+                     features, (long long)channel_stride, (long long)cell_stride);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(SWE_EHIP, std::string("k_features: ") + hipGetErrorString(e));
+  return SWE_OK;
+}
+
+int swe_apply_residual_correction(SweContext* c, SweState predicted, const float* residual,
+                                  int64_t channel_stride, int64_t cell_stride, float relaxation,
+                                  const float* clip_limits, SweState corrected, double* volumes,
+                                  void* stream) {
+  if (!c || !valid(predicted) || !valid(corrected) || !residual)
+    return set_error(SWE_EINVAL, "swe_apply_residual_correction: null argument");
+  const Grid& G = *context_grid(c);
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  const int nb = std::min(grid_blocks(N), context_reduce_blocks(c));
+  double* scratch = context_scratch(c, 2 * (size_t)nb + 4);
+  if (!scratch) return set_error(SWE_EHIP, "swe_apply_residual_correction: scratch allocation failed");
+  Residual R;
+  R.r = residual;
+  R.cs = channel_stride;
+  R.ps = cell_stride;
+  R.relax = relaxation;
+  for (int i = 0; i < 3; ++i) R.lim[i] = clip_limits ? clip_limits[i] : HUGE_VALF;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = scratch + 4;
+  hipLaunchKernelGGL(k_corr_sums, dim3(nb), dim3(kBlock), 0, s, G, predicted.h, R, part);
+  hipLaunchKernelGGL(k_corr_scale, dim3(1), dim3(kBlock), 0, s, part, nb, scratch);
+  hipLaunchKernelGGL(k_corr_apply, dim3(grid_blocks(N)), dim3(kBlock), 0, s, G, predicted.h,
+                     predicted.hu, predicted.hv, R, scratch, corrected.h, corrected.hu,
+                     corrected.hv);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_error(SWE_EHIP, std::string("residual correction: ") + hipGetErrorString(e));
+  if (volumes) {
+    e = hipMemcpyAsync(volumes, scratch, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+      return set_error(SWE_EHIP, std::string("residual correction volumes: ") + hipGetErrorString(e));
+  }
+  return SWE_OK;
+}
+
+}  // extern "C"

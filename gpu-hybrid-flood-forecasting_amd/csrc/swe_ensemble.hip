@@ -6,11 +6,14 @@
 // members' strips (62 columns x `rows` rows each) with a grid stride, so
 // members of 256^2 and 2048^2 cells share one launch and one load balance.
 // Each member keeps its own control block (dt, clock, step count) and its
-// own per-wavefront partials; the wavefront that finishes a member's last
-// strip reduces that member's CFL maximum and FP64 mass terms and plans the
-// member's next step, so no member waits for another.  The host captures
-// chunks of launches in a HIP graph and polls the members' stop flags
-// between chunks.
+// own per-wavefront partials, written with plain stores; after the launch,
+// k_ensemble_finish (one workgroup per running member) reduces each
+// member's CFL maximum and FP64 mass terms and plans that member's next
+// step, so members advance at their own dt.  (A wavefront-level "last one
+// closes the member" handshake needs an agent-scope release per wavefront,
+// which on gfx950 writes back the XCD's L2 each time; the kernel boundary
+// gives the same visibility once per launch.)  The host captures chunks of
+// launches in a HIP graph and polls the members' stop flags between chunks.
 //
 // The arithmetic per member is exactly the single-domain step
 // (swe_strip.hpp / swe_device.hpp), i.e. jax_solver/numerics.py:312-369
@@ -76,7 +79,7 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
        w += stride) {
     const int m = find_member(mem, n_members, w);
     const Member& mb = mem[m];
-    LoopControl* ctl = mb.ctl;
+    const LoopControl* ctl = mb.ctl;
     if (ctl->done) continue;
     const int local = w - mb.wave_base;
     const size_t N = (size_t)mb.G.nx * (size_t)mb.G.ny;
@@ -93,50 +96,44 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
       march_strip<M, true>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
                            local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dt_f, ctl->dtrate_f,
                            ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr, nullptr, nullptr, acc);
+    // Plain stores: the finish kernel reads them after the kernel boundary.
     const float inv = wave_max(acc.inv);
     const double vol = wave_sum(acc.vol);
     const double clip = wave_sum(acc.clip);
     const double bnd = wave_sum(acc.bnd);
     const int nf = wave_or(acc.nf);
-    int last = 0;
     if (lane == 0) {
-      __hip_atomic_store(P.inv + w, inv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(P.vol + w, vol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(P.clip + w, clip, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(P.bnd + w, bnd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(P.nf + w, nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned prev =
-          __hip_atomic_fetch_add(&ctl->counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      last = prev == (unsigned)(mb.n_waves - 1);
+      P.inv[w] = inv;
+      P.vol[w] = vol;
+      P.clip[w] = clip;
+      P.bnd[w] = bnd;
+      P.nf[w] = nf;
     }
-    last = __shfl(last, 0, kWave);
-    if (!last) continue;
-    // This wavefront closes member m's step.
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    float ginv = 0.0f;
-    double gvol = 0.0, gclip = 0.0, gbnd = 0.0;
-    int gnf = 0;
-    for (int k = lane; k < mb.n_waves; k += kWave) {
-      const int q = mb.wave_base + k;
-      ginv = fmaxf(ginv, coherent_load(P.inv + q));
-      gvol += coherent_load(P.vol + q);
-      gclip += coherent_load(P.clip + q);
-      gbnd += coherent_load(P.bnd + q);
-      gnf |= coherent_load(P.nf + q);
-    }
-    Totals tot;
-    tot.inv = wave_max(ginv);
-    tot.vol = wave_sum(gvol);
-    tot.clip = wave_sum(gclip);
-    tot.bnd = wave_sum(gbnd);
-    tot.nf = wave_or(gnf);
-    const double t_new = ctl->t + ctl->dt;
-    int rj = 0;
-    for (int k = lane; k < mb.L.n_rain - 1; k += kWave)
-      if (mb.L.rain_t[k] <= t_new && t_new < mb.L.rain_t[k + 1]) rj = k;
-    rj = wave_max_int(rj);
-    if (lane == 0) finish_step(ctl, mb.L, tot, t_new, rj);
   }
+}
+
+// One workgroup per running member: reduce the member's wavefront partials of
+// the launch that just ended, close its step in FP64 and plan its next one.
+__global__ __launch_bounds__(256) void k_ensemble_finish(const Member* __restrict__ mem,
+                                                        EnsPartials P) {
+  const Member& mb = mem[blockIdx.x];
+  LoopControl* ctl = mb.ctl;
+  if (ctl->done) return;
+  float inv = 0.0f;
+  double vol = 0.0, clip = 0.0, bnd = 0.0;
+  int nf = 0;
+  for (int k = threadIdx.x; k < mb.n_waves; k += 256) {
+    const int q = mb.wave_base + k;
+    inv = fmaxf(inv, P.inv[q]);
+    vol += P.vol[q];
+    clip += P.clip[q];
+    bnd += P.bnd[q];
+    nf |= P.nf[q];
+  }
+  const Totals tot = block_reduce(inv, vol, clip, bnd, nf);
+  const double t_new = ctl->t + ctl->dt;
+  const int rj = rain_bracket(mb.L, t_new);
+  if (threadIdx.x == 0) finish_step(ctl, mb.L, tot, t_new, rj);
 }
 
 }  // namespace swe
@@ -252,6 +249,8 @@ void launch_ens(SweEnsemble* e, hipStream_t s) {
   else
     hipLaunchKernelGGL(k_ensemble_step<FastMath>, dim3(e->grid), dim3(256), 0, s, e->d_mem,
                        (int)e->act.size(), e->total_waves, e->P);
+  hipLaunchKernelGGL(k_ensemble_finish, dim3((unsigned)e->act.size()), dim3(256), 0, s, e->d_mem,
+                     e->P);
 }
 
 int launch_chunks(SweEnsemble* e, int launches, hipStream_t s) {
@@ -423,6 +422,7 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
     L.times = e->d_times + 2 * m;
     L.n_rain = p.n_rainfall;
     L.n_times = 2;
+    L.record = 0;
     for (int k = 0; k < p.n_rainfall; ++k) rain_all.push_back(p.rainfall_times_s[k]);
     for (int k = 0; k < p.n_rainfall; ++k) rain_all.push_back(p.rainfall_rates_m_s[k]);
     rain_off += 2 * (size_t)p.n_rainfall;

@@ -9,10 +9,10 @@
 //   - fused multiply-adds everywhere except the mass flux, whose two products
 //     stay separately rounded so that a wall (mirrored) face carries exactly
 //     zero mass: a*q + b*(-q) with a == b;
-//   - one reciprocal of max(h, dry) shared by the friction law and the CFL
-//     epilogue; x^(7/3) as x^2 * 2^(log2(x)/3 * ...) on the hardware log/exp;
-//   - depth and clip volume summed in float64 per cell, so the totals do not
-//     depend on the strip shape.
+//   - one reciprocal of max(h, dry) for the CFL epilogue; x^(-7/3) of the
+//     friction law as 2^(-7/3 log2 x) on the hardware log/exp;
+//   - depth summed in float64 per cell, so the volume does not depend on the
+//     strip shape; the (small) clip depth per lane in float32.
 // Results stay within the stated tolerance of the FP64 oracle; the lake at
 // rest, wall mass flux and open-boundary accounting keep their exact forms.
 #pragma once
@@ -258,6 +258,7 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
   float nn = io.ld(io.n, j0);
   float mult = io.ld(io.mult, j0);
   float nf_probe = 0.0f;
+  float clip32 = 0.0f;  // clip depth is small; float32 per lane over the strip
 
   // Row j: `cur` is row j, `north` row j+1 (loaded one row ago).  Issues the
   // loads of row j+2 into `ahead` and of row j+1's n/multiplier into
@@ -301,9 +302,9 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
     const float hd = fmaxf(hn, dry);
     const float rhd = __builtin_amdgcn_rcpf(hd);
     const float q = __builtin_amdgcn_sqrtf(fmaf(hus, hus, hvs * hvs));
-    // 1 / hd^(7/3) = (1/hd)^2 * 2^(-log2(hd) * (7/3 - 2)); K = dtg n^2 q / hd^(7/3)
-    const float ip13 = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(hd) * -kPow73MinusTwo);
-    const float K = ((dtg * (nn_j * nn_j)) * q) * ((rhd * rhd) * ip13);
+    // K = dtg n^2 q / hd^(7/3), the power as 2^(-7/3 log2 hd) on the hardware log/exp
+    const float ip73 = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(hd) * -kPow73);
+    const float K = ((dtg * (nn_j * nn_j)) * q) * ip73;
     const float rD = __builtin_amdgcn_rcpf(1.0f + K);
     const bool wet = cur.s.a && (hn > dry);
     const float h_o = cur.s.a ? hn : 0.0f;
@@ -329,8 +330,7 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
         const float inv = fmaf(fabsf(hv_o * rhd) + ws, rdy, (fabsf(hu_o * rhd) + ws) * rdx);
         acc.inv = fmaxf(acc.inv, cur.s.a ? inv : 0.0f);
         acc.vol += (double)h_o;
-        const float clip = cur.s.a ? fmaxf(-hs, 0.0f) : 0.0f;
-        acc.clip += (double)clip;
+        clip32 += cur.s.a ? fmaxf(-hs, 0.0f) : 0.0f;
         // (x + y + z) * 0 stays 0 unless one of them is infinite or NaN
         nf_probe = fmaf(h_o + hu_o + hv_o, 0.0f, nf_probe);
       }
@@ -353,7 +353,10 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
   if (j < j1) row(j, cur, alt, next, next2, nn, mult, nn2, mult2);
   // yt0 now holds the mass flux through the north face of row j1-1.
   if (LOOP && j1 == G.ny && own) acc.bnd -= (double)yt0 * dxd;
-  if (LOOP) acc.nf |= !isfinite(nf_probe);
+  if (LOOP) {
+    acc.nf |= !isfinite(nf_probe);
+    acc.clip += (double)clip32;
+  }
 }
 
 }  // namespace swe

@@ -7,11 +7,15 @@ struct SweContext;
 
 namespace swe {
 struct Grid;
-// Read-only views of a single-domain context for the adjoint's launcher.
+// Views of a single-domain context for the other translation units.
 const Grid* context_grid(const SweContext* c);
 int context_variant(const SweContext* c);
 double context_gravity(const SweContext* c);
 double context_rate(const SweContext* c, double t);
+int context_reduce_blocks(const SweContext* c);
+// Device scratch of at least n_doubles doubles owned by the context (grown on
+// demand, kept until swe_destroy); nullptr if the allocation failed.
+double* context_scratch(SweContext* c, size_t n_doubles);
 
 // Record `msg` as the calling thread's last error (swe_last_error) and return `code`.
 int set_error(int code, const std::string& msg);

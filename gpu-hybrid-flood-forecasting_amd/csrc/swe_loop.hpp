@@ -34,6 +34,7 @@ struct LoopConst {
   const double* times;
   double* dt_log;  // optional: dt of step k is written to dt_log[k] (k < dt_log_cap)
   int n_rain, n_times, max_steps, dt_log_cap;
+  int record;      // 0: output times only stop/clamp the loop, no state is stored
 };
 
 struct Partials {
@@ -74,7 +75,7 @@ __device__ void plan_next_step(LoopControl* ctl, const LoopConst& L, float m, do
   ctl->dtrate_f = (float)(dt * rate);
   ctl->dtg_f = (float)(dt * L.gravity);
   ctl->rec_lo = oi;
-  ctl->rec_hi = hi;
+  ctl->rec_hi = L.record ? hi : oi;
   if (L.dt_log && ctl->steps < L.dt_log_cap) L.dt_log[ctl->steps] = dt;
 }
 
@@ -85,10 +86,11 @@ struct Totals {
   int nf;
 };
 
+template <int WAVES = kWavesPerBlock>
 __device__ Totals block_reduce(float inv, double vol, double clip, double bnd, int nf) {
-  __shared__ float s_inv[kWavesPerBlock];
-  __shared__ double s_vol[kWavesPerBlock], s_clip[kWavesPerBlock], s_bnd[kWavesPerBlock];
-  __shared__ int s_nf[kWavesPerBlock];
+  __shared__ float s_inv[WAVES];
+  __shared__ double s_vol[WAVES], s_clip[WAVES], s_bnd[WAVES];
+  __shared__ int s_nf[WAVES];
   inv = wave_max(inv);
   vol = wave_sum(vol);
   clip = wave_sum(clip);
@@ -105,7 +107,7 @@ __device__ Totals block_reduce(float inv, double vol, double clip, double bnd, i
   __syncthreads();
   Totals t{0.0f, 0.0, 0.0, 0.0, 0};
   if (threadIdx.x == 0) {
-    for (int k = 0; k < kWavesPerBlock; ++k) {
+    for (int k = 0; k < WAVES; ++k) {
       t.inv = fmaxf(t.inv, s_inv[k]);
       t.vol += s_vol[k];
       t.clip += s_clip[k];

@@ -93,21 +93,42 @@ __global__ __launch_bounds__(kBlock) void k_step(
 // CFL maximum / volume of a state; MODE RED_INIT primes the loop.
 // ---------------------------------------------------------------------------
 // Finish of MODE_LOOP_SPLIT: one workgroup reduces the step kernel's
-// partials (visible at the kernel boundary) and plans the next step.
-__global__ __launch_bounds__(kBlock) void k_finish(LoopControl* __restrict__ ctl, LoopConst L,
-                                                  Partials P, int nparts) {
+// partials (visible at the kernel boundary) and plans the next step.  It is
+// on the critical path of every step, so it is wide and issues all its loads
+// before it adds: kFinishUnroll partials per thread per round trip.
+constexpr int kFinishBlock = 1024;
+constexpr int kFinishUnroll = 8;
+
+__global__ __launch_bounds__(kFinishBlock) void k_finish(LoopControl* __restrict__ ctl,
+                                                         LoopConst L, Partials P, int nparts) {
   if (ctl->done) return;
   float inv = 0.0f;
   double vol = 0.0, clip = 0.0, bnd = 0.0;
   int nf = 0;
-  for (int k = threadIdx.x; k < nparts; k += kBlock) {
-    inv = fmaxf(inv, P.inv[k]);
-    vol += P.vol[k];
-    clip += P.clip[k];
-    bnd += P.bnd[k];
-    nf |= P.nf[k];
+  for (int base = threadIdx.x; base < nparts; base += kFinishBlock * kFinishUnroll) {
+    float pi[kFinishUnroll];
+    double pv[kFinishUnroll], pc[kFinishUnroll], pb[kFinishUnroll];
+    int pn[kFinishUnroll];
+#pragma unroll
+    for (int u = 0; u < kFinishUnroll; ++u) {
+      const int k = base + u * kFinishBlock;
+      const bool in = k < nparts;
+      pi[u] = in ? P.inv[k] : 0.0f;
+      pv[u] = in ? P.vol[k] : 0.0;
+      pc[u] = in ? P.clip[k] : 0.0;
+      pb[u] = in ? P.bnd[k] : 0.0;
+      pn[u] = in ? P.nf[k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kFinishUnroll; ++u) {
+      inv = fmaxf(inv, pi[u]);
+      vol += pv[u];
+      clip += pc[u];
+      bnd += pb[u];
+      nf |= pn[u];
+    }
   }
-  const Totals tot = block_reduce(inv, vol, clip, bnd, nf);
+  const Totals tot = block_reduce<kFinishBlock / kWave>(inv, vol, clip, bnd, nf);
   const double t_new = ctl->t + ctl->dt;
   const int rj = rain_bracket(L, t_new);
   if (threadIdx.x == 0) finish_step(ctl, L, tot, t_new, rj);
@@ -186,6 +207,16 @@ __global__ __launch_bounds__(kBlock) void k_sum_mult(Grid G, double* __restrict_
 
 // Copy the loop's final state (buffer A or B by the parity of the step count)
 // to `dst`, so the host need not know how many steps the device took.
+__global__ __launch_bounds__(kBlock) void k_fill3(float* __restrict__ a, float* __restrict__ b,
+                                                 float* __restrict__ c, size_t n, float v) {
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * kBlock) {
+    a[k] = v;
+    b[k] = v;
+    c[k] = v;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy_final(const LoopControl* __restrict__ ctl,
                                                       const float* __restrict__ a,
                                                       const float* __restrict__ b,
@@ -257,6 +288,10 @@ struct SweContext {
   double loop_times[2] = {0.0, 0.0};
   LoopConst loop_L{};
   bool loop_active = false;
+  double* aux = nullptr;  // scratch of the coupled-forecast kernels
+  size_t aux_cap = 0;
+  int lds_per_cu = 0;        // bytes of LDS per compute unit
+  unsigned step_lds = 0;     // dynamic LDS reserved per step workgroup (occupancy cap)
 
   size_t cells() const { return (size_t)G.nx * (size_t)G.ny; }
   float* buf(int which, int field) { return work + ((size_t)which * 3 + field) * cells(); }
@@ -285,6 +320,7 @@ struct SweContext {
     L.n_rain = (int)rain_t.size();
     L.n_times = n_times;
     L.max_steps = max_steps;
+    L.record = 1;
     return L;
   }
 };
@@ -333,7 +369,7 @@ template <class M, int MODE>
 void launch_step(SweContext* c, const float* h, const float* hu, const float* hv, float* ho,
                  float* huo, float* hvo, float dt, float dtrate, float dtg, const LoopConst& L,
                  float* ch, float* chu, float* chv, hipStream_t s) {
-  hipLaunchKernelGGL((k_step<M, MODE>), dim3(c->step_blocks()), dim3(kBlock), 0, s, c->G, h, hu,
+  hipLaunchKernelGGL((k_step<M, MODE>), dim3(c->step_blocks()), dim3(kBlock), c->step_lds, s, c->G, h, hu,
                      hv, ho, huo, hvo, dt, dtrate, dtg, c->rows, c->segs_x(), c->n_waves(),
                      c->d_ctl, L, c->P, ch, chu, chv);
 }
@@ -348,7 +384,8 @@ void launch_step_any(SweContext* c, int mode, const float* h, const float* hu, c
     else
       launch_step<FastMath, MODE_LOOP_SPLIT>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch,
                                              chu, chv, s);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, s, c->d_ctl, L, c->P, c->step_blocks());
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishBlock), 0, s, c->d_ctl, L, c->P,
+                       c->step_blocks());
     return;
   }
   if (c->variant == SWE_VARIANT_REFERENCE) {
@@ -541,6 +578,17 @@ const Grid* swe::context_grid(const SweContext* c) { return &c->G; }
 int swe::context_variant(const SweContext* c) { return c->variant; }
 double swe::context_gravity(const SweContext* c) { return c->gravity; }
 double swe::context_rate(const SweContext* c, double t) { return host_interp(c, t); }
+int swe::context_reduce_blocks(const SweContext* c) { return c->reduce_blocks(); }
+double* swe::context_scratch(SweContext* c, size_t n_doubles) {
+  if (n_doubles > c->aux_cap) {
+    if (c->aux && hipFree(c->aux) != hipSuccess) return nullptr;
+    c->aux = nullptr;
+    c->aux_cap = 0;
+    if (hipMalloc(&c->aux, sizeof(double) * n_doubles) != hipSuccess) return nullptr;
+    c->aux_cap = n_doubles;
+  }
+  return c->aux;
+}
 
 // ===========================================================================
 // C ABI
@@ -598,6 +646,7 @@ int swe_create(const SweParamsDesc* p, int32_t variant, SweContext** out) {
 
   SweContext* c = new SweContext();
   c->device = dev;
+  c->lds_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
   c->variant = variant;
   c->dx = p->dx;
   c->dy = p->dy;
@@ -669,6 +718,7 @@ int swe_destroy(SweContext* c) {
   if (c->d_ctl) (void)hipFree(c->d_ctl);
   if (c->h_ctl) (void)hipHostFree(c->h_ctl);
   if (c->d_rain) (void)hipFree(c->d_rain);
+  if (c->aux) (void)hipFree(c->aux);
   delete c;
   return SWE_OK;
 }
@@ -686,6 +736,24 @@ int swe_set_tile_rows(SweContext* c, int32_t rows) {
   if (rows < 0 || rows > 4096) return fail(SWE_EINVAL, "swe_set_tile_rows: rows must be in [0, 4096]");
   c->rows = rows == 0 ? kDefaultRows : rows;
   return ensure_partials(c);
+}
+
+int swe_set_waves_per_cu(SweContext* c, int32_t waves) {
+  if (!c) return fail(SWE_EINVAL, "swe_set_waves_per_cu: null context");
+  if (waves < 0 || (waves > 0 && (waves % kWavesPerBlock != 0 || waves < 2 * kWavesPerBlock)))
+    return fail(SWE_EINVAL, "swe_set_waves_per_cu: waves must be 0 or a multiple of 4 >= 8");
+  if (waves == 0 || c->lds_per_cu <= 0) {
+    c->step_lds = 0;
+    return SWE_OK;
+  }
+  // Reserve enough LDS per workgroup that at most waves/4 workgroups fit on a
+  // compute unit (one more would exceed its LDS).
+  const int blocks = waves / kWavesPerBlock;
+  const unsigned lds = (unsigned)(c->lds_per_cu / (blocks + 1) + 512);
+  if ((long)lds * blocks > (long)c->lds_per_cu - 1024 || lds > 65536)
+    return fail(SWE_EINVAL, "swe_set_waves_per_cu: cap not reachable with this LDS size");
+  c->step_lds = lds;
+  return SWE_OK;
 }
 
 int swe_set_graph_chunk(SweContext* c, int32_t chunk) {
@@ -823,6 +891,47 @@ int swe_run_to_outputs(SweContext* c, SweState initial, const double* times, int
                   max_steps, k.out_idx, n_times, k.t);
     return fail(SWE_EMAXSTEPS, msg);
   }
+  return SWE_OK;
+}
+
+int swe_advance_to_time(SweContext* c, SweState in, double t_start, double t_target,
+                        int32_t max_substeps, SweState out, int32_t* steps_used, double* t_final,
+                        void* stream) {
+  if (!c || !valid_state(in) || !valid_state(out))
+    return fail(SWE_EINVAL, "swe_advance_to_time: null argument");
+  if (!std::isfinite(t_start) || !std::isfinite(t_target))
+    return fail(SWE_EINVAL, "swe_advance_to_time: times must be finite");
+  if (max_substeps < 0) return fail(SWE_EINVAL, "swe_advance_to_time: max_substeps must be >= 0");
+  if (c->loop_active) return fail(SWE_EINVAL, "swe_advance_to_time: a device loop is active");
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  if ((rc = ensure_work(c)) || (rc = ensure_partials(c)) || (rc = ensure_times(c, 2))) return rc;
+  double times[2] = {t_start, t_target};
+  SWE_HIP_TRY(hipMemcpyAsync(c->d_times, times, sizeof times, hipMemcpyHostToDevice, s));
+  if ((rc = copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), in.h, in.hu, in.hv, s)))
+    return rc;
+  LoopConst L = c->loop_const(2, max_substeps);
+  L.record = 0;
+  launch_reduce(c, RED_INIT, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), L, s);
+  SWE_HIP_TRY(hipGetLastError());
+  // The time arguments stay alive until the copy above has run.
+  SWE_HIP_TRY(hipStreamSynchronize(s));
+  if ((rc = drive_loop(c, L, nullptr, nullptr, nullptr, s, max_substeps))) return rc;
+  const LoopControl k = c->h_ctl[0];
+  const int fin = k.steps & 1;  // launch n reads buffer n&1 and writes the other
+  if ((rc = copy_state(c, out.h, out.hu, out.hv, c->buf(fin, 0), c->buf(fin, 1), c->buf(fin, 2), s)))
+    return rc;
+  // coupled_forecast.py:152-156: a target not reached within 1e-6 s gives NaN.
+  if (!(k.t >= t_target - 1.0e-6)) {
+    const size_t N = c->cells();
+    const int nb = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_fill3, dim3(nb), dim3(kBlock), 0, s, out.h, out.hu, out.hv, N,
+                       __builtin_nanf(""));
+    SWE_HIP_TRY(hipGetLastError());
+  }
+  if (steps_used) *steps_used = k.steps;
+  if (t_final) *t_final = k.t;
+  SWE_HIP_TRY(hipStreamSynchronize(s));
   return SWE_OK;
 }
 

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_int64, c_void_p
 
 import torch  # noqa: F401  (load torch's HIP runtime before the library)
 
@@ -89,6 +89,7 @@ SIGNATURES = {
     "swe_destroy": (c_int32, [c_void_p]),
     "swe_set_variant": (c_int32, [c_void_p, c_int32]),
     "swe_set_tile_rows": (c_int32, [c_void_p, c_int32]),
+    "swe_set_waves_per_cu": (c_int32, [c_void_p, c_int32]),
     "swe_set_graph_chunk": (c_int32, [c_void_p, c_int32]),
     "swe_set_dt_log": (c_int32, [c_void_p, c_void_p, c_int32]),
     "swe_set_loop_finish": (c_int32, [c_void_p, c_int32]),
@@ -113,6 +114,26 @@ SIGNATURES = {
         c_int32,
         [c_void_p, SweState, POINTER(c_double), c_int32, c_int32, c_void_p, c_void_p, c_void_p,
          SweState, POINTER(SweRunInfo), c_void_p],
+    ),
+    "swe_advance_to_time": (
+        c_int32,
+        [c_void_p, SweState, c_double, c_double, c_int32, SweState, POINTER(c_int32),
+         POINTER(c_double), c_void_p],
+    ),
+    "swe_network_features": (
+        c_int32,
+        [c_void_p, SweState, c_void_p, c_void_p, POINTER(c_float), POINTER(c_float), c_double,
+         c_double, c_void_p, c_int64, c_int64, c_void_p],
+    ),
+    "swe_apply_residual_correction": (
+        c_int32,
+        [c_void_p, SweState, c_void_p, c_int64, c_int64, c_float, POINTER(c_float), SweState,
+         POINTER(c_double), c_void_p],
+    ),
+    "swe_conserved_to_primitive": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p],
     ),
     "swe_run_steps": (
         c_int32, [c_void_p, SweState, c_int32, c_double, POINTER(SweRunInfo), c_void_p]
@@ -230,6 +251,9 @@ class Context:
 
     def set_tile_rows(self, rows: int) -> None:
         check(load().swe_set_tile_rows(self.handle, int(rows)), "swe_set_tile_rows")
+
+    def set_waves_per_cu(self, waves: int) -> None:
+        check(load().swe_set_waves_per_cu(self.handle, int(waves)), "swe_set_waves_per_cu")
 
     def set_loop_finish(self, split: int) -> None:
         check(load().swe_set_loop_finish(self.handle, int(split)), "swe_set_loop_finish")

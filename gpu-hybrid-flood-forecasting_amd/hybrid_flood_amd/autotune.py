@@ -1,10 +1,13 @@
-"""Tile-shape autotuner: rows per wavefront strip of the fused step.
+"""Tile-shape autotuner: rows per wavefront strip of the fused step, and the
+number of resident wavefronts per compute unit.
 
 A strip is 62 columns wide (one wavefront) and ``rows`` tall.  Taller strips
 amortise the halo row and the strip prologue; shorter strips give more
 wavefronts to hide memory latency and a smaller tail.  The best value depends
-on the grid shape and the kernel variant, so it is measured: each candidate
-runs a short device loop on a copy of the caller's state, timed with HIP
+on the grid shape and the kernel variant, so it is measured, together with a
+cap on resident wavefronts per compute unit (fewer concurrent row streams can
+use HBM better): each candidate pair runs a short device loop on a copy of
+the caller's state, timed with HIP
 events, and the winner is cached in a JSON file keyed by device
 architecture, grid shape and variant (``$SWE_TUNE_CACHE`` or
 ``~/.cache/hybrid_flood_amd/tuning.json``).  Every candidate gives
@@ -23,6 +26,8 @@ from . import _lib
 from .numerics import SWEParams, SWEState, context_for
 
 CANDIDATE_ROWS = (8, 12, 16, 24, 32, 48, 64)
+# Resident wavefronts per compute unit: 0 = as many as registers allow.
+CANDIDATE_WAVES = (0, 20, 16)
 
 
 def cache_path() -> Path:
@@ -75,26 +80,31 @@ def time_loop_rows(ctx, state: SWEState, rows: int, steps: int) -> float:
 
 
 def tune_rows(params: SWEParams, state: SWEState, *, variant: str | None = None, steps: int = 32,
-              candidates=CANDIDATE_ROWS, use_cache: bool = True) -> dict:
-    """Pick and apply the fastest strip height for this grid; returns the record."""
+              candidates=CANDIDATE_ROWS, waves=CANDIDATE_WAVES, use_cache: bool = True) -> dict:
+    """Pick and apply the fastest (strip height, waves per CU) for this grid; returns the record."""
     from .numerics import DEFAULT_VARIANT
 
     variant = variant or DEFAULT_VARIANT
     ctx = context_for(params, variant)
-    key = _key("step", tuple(state.h.shape), variant)
+    key = _key("step2", tuple(state.h.shape), variant)
     cache = _load_cache() if use_cache else {}
     rec = cache.get(key)
     if rec is None:
         chunk_steps = max(2, steps - steps % 2)
         ctx.set_graph_chunk(chunk_steps)
-        timings = {int(r): time_loop_rows(ctx, state, int(r), chunk_steps) for r in candidates}
+        timings = {}
+        for w in waves:
+            ctx.set_waves_per_cu(int(w))
+            for r in candidates:
+                timings[(int(r), int(w))] = time_loop_rows(ctx, state, int(r), chunk_steps)
         best = min(timings, key=timings.get)
-        rec = {"rows": best, "ms_per_step": timings[best],
-               "candidates": {str(k): v for k, v in timings.items()}}
+        rec = {"rows": best[0], "waves_per_cu": best[1], "ms_per_step": timings[best],
+               "candidates": {f"{r}x{w}": v for (r, w), v in timings.items()}}
         if use_cache:
             cache[key] = rec
             _store_cache(cache)
     ctx.set_tile_rows(int(rec["rows"]))
+    ctx.set_waves_per_cu(int(rec.get("waves_per_cu", 0)))
     return rec
 
 

@@ -108,6 +108,11 @@ int swe_destroy(SweContext* ctx);
 int swe_set_variant(SweContext* ctx, int32_t variant);
 /* Rows per wavefront strip of the fused step (tuning knob; 0 = default). */
 int swe_set_tile_rows(SweContext* ctx, int32_t rows);
+/* Cap the fused step at `waves` resident wavefronts per compute unit by
+ * reserving LDS per workgroup (tuning knob: fewer concurrent row streams can
+ * raise HBM efficiency).  0 = no cap; else a multiple of 4, >= 8.  No
+ * reference counterpart (the reference has no tiling). */
+int swe_set_waves_per_cu(SweContext* ctx, int32_t waves);
 /* How the device loop closes a step (tuning knob): 1 (default) = every
  * workgroup stores its CFL/mass partials and a one-workgroup kernel reduces
  * them and plans the next step; 0 = the last workgroup of the step kernel does
@@ -154,6 +159,18 @@ int swe_interp_rate(const float* times_s, const float* rates_m_s, int32_t n, dou
 int swe_run_to_outputs(SweContext* ctx, SweState initial, const double* output_times_s,
                        int32_t n_times, int32_t max_steps, float* out_h, float* out_hu,
                        float* out_hv, SweState final_state, SweRunInfo* info, void* stream);
+
+/* Advance adaptively from t_start to t_target in at most max_substeps CFL
+ * steps (hybrid/coupled_forecast.py:123-161, advance_physics_to_time): while
+ * t < t_target - 1e-7, dt = min(CFL dt, t_target - t).  `out` receives the
+ * final state, or NaN everywhere when the clock ended more than 1e-6 s short
+ * of t_target (coupled_forecast.py:152-156); `in` and `out` may alias.
+ * *steps_used / *t_final (either may be NULL) receive the step count and the
+ * float64 clock.  Runs the device loop; synchronises the stream.  A target at
+ * or before t_start takes no step. */
+int swe_advance_to_time(SweContext* ctx, SweState in, double t_start, double t_target,
+                        int32_t max_substeps, SweState out, int32_t* steps_used, double* t_final,
+                        void* stream);
 
 /* Exactly n_steps adaptive steps (dt = CFL dt capped at max_dt, no output
  * target) from time t0, state updated in place.  The device loop of
@@ -261,6 +278,50 @@ int swe_ensemble_run(SweEnsemble* ens, double t_end, int32_t max_steps, SweRunIn
                      void* stream);
 /* Strips (wavefronts of work) per launch and the persistent grid size. */
 int swe_ensemble_total_waves(const SweEnsemble* ens, int32_t* waves, int32_t* grid);
+
+/* ---- Hybrid coupled forecast (hybrid/coupled_forecast.py) ---------------- */
+
+/* network_features (coupled_forecast.py:53-79): the six normalised CNN input
+ * channels of `state` -- (h, u, v) standardised with state_mean/state_std
+ * (host float32[3]) and zeroed outside the domain mask, the rain channel
+ * float32(rate(time_s) * 3.6e6) / rainfall_scale_mm_hr (rate from the
+ * context's series, as swe_rainfall_rate), terrain_std and roughness_std
+ * (device float32[ny][nx]).  Channel c of cell k (row-major) is written to
+ * features[c * channel_stride + k * cell_stride]: NHWC is channel_stride 1,
+ * cell_stride 6; NCHW is channel_stride N, cell_stride 1.  Asynchronous. */
+int swe_network_features(SweContext* ctx, SweState state, const float* terrain_std,
+                         const float* roughness_std, const float* state_mean,
+                         const float* state_std, double rainfall_scale_mm_hr, double time_s,
+                         float* features, int64_t channel_stride, int64_t cell_stride,
+                         void* stream);
+
+/* apply_residual_correction (coupled_forecast.py:82-120) with the residual
+ * clip of coupled_forecast.py:199-201 fused in.  residual channel c of cell
+ * k is residual[c * channel_stride + k * cell_stride]; each channel is
+ * clipped to +-clip_limits[c] (host float32[3]; NULL = no clip).  The
+ * provisional depth max(h + relaxation*r0, 0) (0 outside the mask) is
+ * rescaled to the masked physics volume with float64 sums; velocities are
+ * corrected by relaxation*r1, relaxation*r2 and re-multiplied where wet.
+ * `corrected` may alias `predicted`.  volumes (optional host double[4])
+ * receives (physics volume, provisional volume, scale, 1 if rescaled) in
+ * cell-depth units and synchronises the stream; otherwise asynchronous. */
+int swe_apply_residual_correction(SweContext* ctx, SweState predicted, const float* residual,
+                                  int64_t channel_stride, int64_t cell_stride, float relaxation,
+                                  const float* clip_limits, SweState corrected, double* volumes,
+                                  void* stream);
+
+/* ---- Output schema (jax_solver/shallow_water_2d.py:309-350,
+ *      structured_io.py:160-211) ------------------------------------------ */
+
+/* Primitive fields of n_times stacked states (device float32
+ * [n_times][ny][nx] each): depth = h; x/y velocity = hu/h, hv/h where
+ * h > dry_tolerance, else 0; velocity = hypot(u, v); all four NaN outside the
+ * domain mask.  elevation (optional, device float32[ny][nx]) = bed, NaN
+ * outside the mask.  Asynchronous. */
+int swe_conserved_to_primitive(SweContext* ctx, const float* h, const float* hu, const float* hv,
+                               int32_t n_times, float* depth, float* x_velocity,
+                               float* y_velocity, float* velocity, float* elevation,
+                               void* stream);
 
 #ifdef __cplusplus
 }

@@ -397,3 +397,30 @@ def run_steps(initial, params, n_steps, *, t0=0.0, norm="sqrt"):
         dts.append(dt)
         t += dt
     return state, t, np.asarray(dts)
+
+
+def advance_physics_to_time(initial, params, start_time_s, target_time_s, *, max_substeps,
+                            norm="sqrt"):
+    """Fixed-length adaptive advance to one target (hybrid/coupled_forecast.py:123-161).
+
+    ``max_substeps`` iterations of: if ``t < target - 1e-7`` take
+    ``dt = min(cfl_dt, target - t)`` and step, else hold.  A clock that ends
+    more than 1e-6 s short of the target gives an all-NaN state
+    (coupled_forecast.py:152-156).  float64 clock, as in the rest of this
+    oracle.  Returns ``(state, steps, t)``.
+    """
+    T = params.dtype
+    state = OracleState(*(np.asarray(x, T) for x in initial))
+    t = float(start_time_s)
+    target = float(target_time_s)
+    steps = 0
+    for _ in range(int(max_substeps)):
+        if not t < target - OUTPUT_TIME_SLACK_S:
+            continue
+        dt = min(compute_cfl_dt(state, params), target - t)
+        state = finite_volume_step(state, params, dt, t, norm=norm)
+        t = t + dt
+        steps += 1
+    if not t >= target - 1.0e-6:
+        state = OracleState(*(np.full_like(x, np.nan) for x in state))
+    return state, steps, t
