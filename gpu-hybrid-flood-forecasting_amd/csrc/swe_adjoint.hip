@@ -30,13 +30,18 @@ namespace swe {
 
 constexpr int kAdjCols = kWave - 4;  // columns completed per wavefront
 
-template <class M>
+// M: arithmetic of the forward recompute (the forward kernel's own, so every
+// branch is the one the forward took).  T: arithmetic and storage of the
+// cotangents and sensitivities -- float, or double, which gfx950 runs at the
+// float rate for add/mul/FMA and which keeps the reverse sweep accurate where
+// the cotangent is a small difference of large terms (transcritical cells).
+template <class M, class T>
 __global__ __launch_bounds__(256) void k_adjoint_step(
     Grid G, const float* __restrict__ H, const float* __restrict__ HU,
-    const float* __restrict__ HV, const float* __restrict__ LH, const float* __restrict__ LHU,
-    const float* __restrict__ LHV, float* __restrict__ OH, float* __restrict__ OHU,
-    float* __restrict__ OHV, float* __restrict__ grad_n, float* __restrict__ grad_mult, float dt,
-    float dtrate, float dtg, int rows, int segs_x, int n_waves) {
+    const float* __restrict__ HV, const T* __restrict__ LH, const T* __restrict__ LHU,
+    const T* __restrict__ LHV, T* __restrict__ OH, T* __restrict__ OHU, T* __restrict__ OHV,
+    T* __restrict__ grad_n, T* __restrict__ grad_mult, float dt, float dtrate, float dtg,
+    int rows, int segs_x, int n_waves) {
   const int gw = blockIdx.x * (blockDim.x / kWave) + wave_in_block();
   if (gw >= n_waves) return;
   const int lane = threadIdx.x & (kWave - 1);
@@ -58,21 +63,21 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
   rd.refl_north = refl_n;
   const int j0 = sy * rows;
   const int j1 = min(j0 + rows, G.ny);
-  const float cx = dt * G.rdx, cy = dt * G.rdy;
-  const float g = G.g, hg = G.half_g, dry = G.dry;
+  const T cx = T(dt) * T(G.rdx), cy = T(dt) * T(G.rdy);
+  const T g = G.g, hg = G.half_g, dry = G.dry;
 
   Cell Rprev = rd.load(G, j0 - 2);
   Cell Rcur = rd.load(G, j0 - 1);
   FaceFlux Gs = face_flux<M, false>(G, Rprev.h, Rprev.v, Rprev.u, Rprev.b, Rprev.a, Rcur.h, Rcur.v,
                                     Rcur.u, Rcur.b, Rcur.a);
-  float lsp0 = 0.0f, lsp1 = 0.0f, lsp2 = 0.0f;  // lambda* of row r-1
-  float lup0 = 0.0f, lup1 = 0.0f, lup2 = 0.0f;  // lambda of row r-1 (partial)
+  T lsp0 = T(0), lsp1 = T(0), lsp2 = T(0);  // lambda* of row r-1
+  T lup0 = T(0), lup1 = T(0), lup2 = T(0);  // lambda of row r-1 (partial)
 
   for (int r = j0 - 1; r <= j1; ++r) {
     const bool valid_row = r >= 0 && r < G.ny;
     const Cell Rnext = rd.load(G, r + 1);
-    float ls0 = 0.0f, ls1 = 0.0f, ls2 = 0.0f;
-    float lu0 = 0.0f, lu1 = 0.0f, lu2 = 0.0f;
+    T ls0 = T(0), ls1 = T(0), ls2 = T(0);
+    T lu0 = T(0), lu1 = T(0), lu2 = T(0);
     // Forward flux of the face r+1/2; needed as row r+1's south face even when
     // row r itself is the south ghost.
     const FaceFlux Gn = face_flux<M, false>(G, Rcur.h, Rcur.v, Rcur.u, Rcur.b, Rcur.a, Rnext.h,
@@ -99,9 +104,9 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
       const float mult = G.mult[idx];
       hs = hs + dtrate * mult;
       // ---- reverse of the pointwise part ----
-      const adj::PointVjp<float> pv =
-          adj::point_vjp<float>(dry, kPow73, hs, hus, hvs, G.n[idx], Rcur.a, dtrate, dtg,
-                                LH[idx], LHU[idx], LHV[idx]);
+      const adj::PointVjp<T> pv =
+          adj::point_vjp<T>(dry, T(kPow73), T(hs), T(hus), T(hvs), T(G.n[idx]), Rcur.a,
+                            T(dtrate), T(dtg), LH[idx], LHU[idx], LHV[idx]);
       if (real) {
         ls0 = pv.lh;
         ls1 = pv.lhu;
@@ -115,21 +120,21 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
       lu1 = ls1;
       lu2 = ls2;
       // ---- reverse of this lane's west x face (A = west cell, B = this cell) ----
-      const float lw0 = from_west(ls0), lw1 = from_west(ls1), lw2 = from_west(ls2);
-      adj::SideGrad<float> dA, dB;
-      adj::face_vjp<true, float>(g, hg, dry, wh, whu, whv, wb, wa, Rcur.h, Rcur.hu, Rcur.hv,
-                                 Rcur.b, Rcur.a, cx * (ls0 - lw0), cx * (ls1 - lw1),
-                                 cx * (ls2 - lw2), -cx * lw1, cx * ls1, dA, dB);
+      const T lw0 = from_west(ls0), lw1 = from_west(ls1), lw2 = from_west(ls2);
+      adj::SideGrad<T> dA, dB;
+      adj::face_vjp<true, T>(g, hg, dry, T(wh), T(whu), T(whv), T(wb), wa, T(Rcur.h),
+                             T(Rcur.hu), T(Rcur.hv), T(Rcur.b), Rcur.a, cx * (ls0 - lw0),
+                             cx * (ls1 - lw1), cx * (ls2 - lw2), -cx * lw1, cx * ls1, dA, dB);
       // x: (h, q_n, q_t) = (h, hu, hv)
       lu0 += dB.h;
       lu1 += dB.qn;
       lu2 += dB.qt;
-      float sw0 = dA.h, sw1 = dA.qn, sw2 = dA.qt;
+      T sw0 = dA.h, sw1 = dA.qn, sw2 = dA.qt;
       if (i == 0) {  // west cell is the left ghost: it copies this cell
         lu0 += dA.h;
         lu1 += refl_w ? -dA.qn : dA.qn;
         lu2 += dA.qt;
-        sw0 = sw1 = sw2 = 0.0f;
+        sw0 = sw1 = sw2 = T(0);
       }
       if (i == G.nx) {  // this lane is the right ghost: its share belongs to cell nx-1
         sw0 += dB.h;
@@ -142,12 +147,12 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
     }
     // ---- reverse of the y face r-1/2 (A = row r-1, B = row r) ----
     if (r >= j0) {
-      adj::SideGrad<float> dA, dB;
+      adj::SideGrad<T> dA, dB;
       // y: normal momentum hv, tangential hu
-      adj::face_vjp<false, float>(g, hg, dry, Rprev.h, Rprev.hv, Rprev.hu, Rprev.b, Rprev.a,
-                                  Rcur.h, Rcur.hv, Rcur.hu, Rcur.b, Rcur.a, cy * (ls0 - lsp0),
-                                  cy * (ls2 - lsp2), cy * (ls1 - lsp1), -cy * lsp2, cy * ls2, dA,
-                                  dB);
+      adj::face_vjp<false, T>(g, hg, dry, T(Rprev.h), T(Rprev.hv), T(Rprev.hu), T(Rprev.b),
+                              Rprev.a, T(Rcur.h), T(Rcur.hv), T(Rcur.hu), T(Rcur.b), Rcur.a,
+                              cy * (ls0 - lsp0), cy * (ls2 - lsp2), cy * (ls1 - lsp1),
+                              -cy * lsp2, cy * ls2, dA, dB);
       if (r - 1 < 0) {  // row r-1 is the south ghost: it copies row 0 (= row r)
         lu0 += dA.h;
         lu1 += dA.qt;
@@ -194,29 +199,30 @@ constexpr int kAdjRows = 16;
 }
 
 namespace {
-int launch_adjoint(SweContext* c, const float* h, const float* hu, const float* hv,
-                   const float* lh, const float* lhu, const float* lhv, float* oh, float* ohu,
-                   float* ohv, double dt, double time_s, float* grad_manning, float* grad_mult,
-                   hipStream_t s);
+template <class T>
+int launch_adjoint(SweContext* c, const float* h, const float* hu, const float* hv, const T* lh,
+                   const T* lhu, const T* lhv, T* oh, T* ohu, T* ohv, double dt, double time_s,
+                   T* grad_manning, T* grad_mult, hipStream_t s);
 }
 
-extern "C" int swe_adjoint_step(SweContext* c, SweState state, SweState lam_next, SweState lam_out,
-                                double dt, double time_s, float* grad_manning, float* grad_mult,
-                                void* stream) {
+namespace {
+template <class T, class S>
+int adjoint_step_any(SweContext* c, SweState state, S lam_next, S lam_out, double dt,
+                     double time_s, T* grad_manning, T* grad_mult, void* stream) {
   if (!c || !state.h || !state.hu || !state.hv || !lam_next.h || !lam_next.hu || !lam_next.hv ||
       !lam_out.h || !lam_out.hu || !lam_out.hv)
     return set_error(SWE_EINVAL, "swe_adjoint_step: null argument");
   if (lam_out.h == lam_next.h || lam_out.hu == lam_next.hu || lam_out.hv == lam_next.hv)
     return set_error(SWE_EINVAL, "swe_adjoint_step: lam_out must not alias lam_next");
-  return launch_adjoint(c, state.h, state.hu, state.hv, lam_next.h, lam_next.hu, lam_next.hv,
-                        lam_out.h, lam_out.hu, lam_out.hv, dt, time_s, grad_manning, grad_mult,
-                        (hipStream_t)stream);
+  return launch_adjoint<T>(c, state.h, state.hu, state.hv, lam_next.h, lam_next.hu, lam_next.hv,
+                           lam_out.h, lam_out.hu, lam_out.hv, dt, time_s, grad_manning, grad_mult,
+                           (hipStream_t)stream);
 }
 
-extern "C" int swe_adjoint_sweep(SweContext* c, const float* states_h, const float* states_hu,
-                                 const float* states_hv, int32_t n, const double* dts,
-                                 const double* ts, SweState lam, SweState lam_tmp,
-                                 float* grad_manning, float* grad_mult, void* stream) {
+template <class T, class S>
+int adjoint_sweep_any(SweContext* c, const float* states_h, const float* states_hu,
+                      const float* states_hv, int32_t n, const double* dts, const double* ts,
+                      S lam, S lam_tmp, T* grad_manning, T* grad_mult, void* stream) {
   if (!c || n < 0 || (n > 0 && (!states_h || !states_hu || !states_hv || !dts || !ts)) || !lam.h ||
       !lam.hu || !lam.hv || !lam_tmp.h || !lam_tmp.hu || !lam_tmp.hv || lam.h == lam_tmp.h)
     return set_error(SWE_EINVAL, "swe_adjoint_sweep: bad arguments");
@@ -225,16 +231,16 @@ extern "C" int swe_adjoint_sweep(SweContext* c, const float* states_h, const flo
   hipStream_t s = (hipStream_t)stream;
   // lambda ping-pongs between lam and lam_tmp; an odd count ends in lam_tmp,
   // so copy back once at the end.
-  SweState a = lam, b = lam_tmp;
+  S a = lam, b = lam_tmp;
   for (int k = n - 1; k >= 0; --k) {
-    int rc = launch_adjoint(c, states_h + (size_t)k * N, states_hu + (size_t)k * N,
-                            states_hv + (size_t)k * N, a.h, a.hu, a.hv, b.h, b.hu, b.hv, dts[k],
-                            ts[k], grad_manning, grad_mult, s);
+    int rc = launch_adjoint<T>(c, states_h + (size_t)k * N, states_hu + (size_t)k * N,
+                               states_hv + (size_t)k * N, a.h, a.hu, a.hv, b.h, b.hu, b.hv, dts[k],
+                               ts[k], grad_manning, grad_mult, s);
     if (rc) return rc;
     std::swap(a, b);
   }
   if (a.h != lam.h) {
-    const size_t bytes = sizeof(float) * N;
+    const size_t bytes = sizeof(T) * N;
     if (hipMemcpyAsync(lam.h, a.h, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
         hipMemcpyAsync(lam.hu, a.hu, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
         hipMemcpyAsync(lam.hv, a.hv, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
@@ -242,15 +248,43 @@ extern "C" int swe_adjoint_sweep(SweContext* c, const float* states_h, const flo
   }
   return SWE_OK;
 }
+}  // namespace
+
+extern "C" int swe_adjoint_step(SweContext* c, SweState state, SweState lam_next, SweState lam_out,
+                                double dt, double time_s, float* grad_manning, float* grad_mult,
+                                void* stream) {
+  return adjoint_step_any<float>(c, state, lam_next, lam_out, dt, time_s, grad_manning, grad_mult,
+                                 stream);
+}
+
+extern "C" int swe_adjoint_step_f64(SweContext* c, SweState state, SweStateF64 lam_next,
+                                    SweStateF64 lam_out, double dt, double time_s,
+                                    double* grad_manning, double* grad_mult, void* stream) {
+  return adjoint_step_any<double>(c, state, lam_next, lam_out, dt, time_s, grad_manning,
+                                  grad_mult, stream);
+}
+
+extern "C" int swe_adjoint_sweep(SweContext* c, const float* states_h, const float* states_hu,
+                                 const float* states_hv, int32_t n, const double* dts,
+                                 const double* ts, SweState lam, SweState lam_tmp,
+                                 float* grad_manning, float* grad_mult, void* stream) {
+  return adjoint_sweep_any<float>(c, states_h, states_hu, states_hv, n, dts, ts, lam, lam_tmp,
+                                  grad_manning, grad_mult, stream);
+}
+
+extern "C" int swe_adjoint_sweep_f64(SweContext* c, const float* states_h, const float* states_hu,
+                                     const float* states_hv, int32_t n, const double* dts,
+                                     const double* ts, SweStateF64 lam, SweStateF64 lam_tmp,
+                                     double* grad_manning, double* grad_mult, void* stream) {
+  return adjoint_sweep_any<double>(c, states_h, states_hu, states_hv, n, dts, ts, lam, lam_tmp,
+                                   grad_manning, grad_mult, stream);
+}
 
 namespace {
-int launch_adjoint(SweContext* c, const float* h, const float* hu, const float* hv,
-                   const float* lh, const float* lhu, const float* lhv, float* oh, float* ohu,
-                   float* ohv, double dt, double time_s, float* grad_manning, float* grad_mult,
-                   hipStream_t s) {
-  SweState state{const_cast<float*>(h), const_cast<float*>(hu), const_cast<float*>(hv)};
-  SweState lam_next{const_cast<float*>(lh), const_cast<float*>(lhu), const_cast<float*>(lhv)};
-  SweState lam_out{oh, ohu, ohv};
+template <class T>
+int launch_adjoint(SweContext* c, const float* h, const float* hu, const float* hv, const T* lh,
+                   const T* lhu, const T* lhv, T* oh, T* ohu, T* ohv, double dt, double time_s,
+                   T* grad_manning, T* grad_mult, hipStream_t s) {
   const Grid& G = *context_grid(c);
   const double rate = context_rate(c, time_s);
   const float dt_f = (float)dt, dtrate_f = (float)(dt * rate);
@@ -259,14 +293,12 @@ int launch_adjoint(SweContext* c, const float* h, const float* hu, const float* 
   const int n_waves = segs_x * ((G.ny + kAdjRows - 1) / kAdjRows);
   const int blocks = (n_waves + 3) / 4;
   if (context_variant(c) == SWE_VARIANT_REFERENCE)
-    hipLaunchKernelGGL(k_adjoint_step<RefMath>, dim3(blocks), dim3(256), 0, s, G, state.h,
-                       state.hu, state.hv, lam_next.h, lam_next.hu, lam_next.hv, lam_out.h,
-                       lam_out.hu, lam_out.hv, grad_manning, grad_mult, dt_f, dtrate_f, dtg_f,
+    hipLaunchKernelGGL((k_adjoint_step<RefMath, T>), dim3(blocks), dim3(256), 0, s, G, h, hu, hv,
+                       lh, lhu, lhv, oh, ohu, ohv, grad_manning, grad_mult, dt_f, dtrate_f, dtg_f,
                        kAdjRows, segs_x, n_waves);
   else
-    hipLaunchKernelGGL(k_adjoint_step<FastMath>, dim3(blocks), dim3(256), 0, s, G, state.h,
-                       state.hu, state.hv, lam_next.h, lam_next.hu, lam_next.hv, lam_out.h,
-                       lam_out.hu, lam_out.hv, grad_manning, grad_mult, dt_f, dtrate_f, dtg_f,
+    hipLaunchKernelGGL((k_adjoint_step<FastMath, T>), dim3(blocks), dim3(256), 0, s, G, h, hu, hv,
+                       lh, lhu, lhv, oh, ohu, ohv, grad_manning, grad_mult, dt_f, dtrate_f, dtg_f,
                        kAdjRows, segs_x, n_waves);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(SWE_EHIP, std::string("swe_adjoint_step: ") + hipGetErrorString(e));

@@ -15,7 +15,10 @@
 // The step size is data (a recorded schedule), not differentiated.  Kinks
 // take the branch the forward took: max(x, 0) passes the cotangent where
 // x > 0, min/max of the wave speeds pass it to the selected argument, the
-// HLL region (left/right upwind or star) follows the forward's speeds.
+// HLL region (left/right upwind or star) follows the forward's speeds.  One
+// deliberate departure from the exact derivative: the wave speed of a wet
+// cell's side reconstructed to 0 < hr <= dry is not differentiated (see
+// face_vjp).
 //
 // Templated on the scalar type: float on the GPU, double in the host build
 // that checks them against finite differences.  No HIP header is needed.
@@ -166,8 +169,16 @@ SWE_HD inline void face_vjp(T g, T hg, T dry, T hA, T qnA, T qtA, T bA, int aA, 
     gQtB += gFB[2] * vnB;
     gvnB += gFB[2] * QtB;
   }
-  if (cA > T(0)) ghrA += gcsA * g / (T(2) * cA);
-  if (cB > T(0)) ghrB += gcsB * g / (T(2) * cB);
+  // d sqrt(g hr)/d hr = g / (2c) is unbounded as hr -> 0+.  When a wet cell
+  // (h > dry) is reconstructed to a dry side (hr <= dry) -- its water level
+  // within the dry tolerance of the neighbour's higher bed, i.e. of the kink
+  // of hr = max(eta - bf, 0) -- the wave speed is held fixed: the exact
+  // derivative there holds only for perturbations smaller than hr and,
+  // accumulated over a rollout, swamps the gradient that finite differences
+  // of any practical size see.  Shallow cells (h <= dry) keep it: their thin
+  // films change smoothly with the forcing.
+  if (cA > T(0) && (wA || !dwA)) ghrA += gcsA * g / (T(2) * cA);
+  if (cB > T(0) && (wB || !dwB)) ghrB += gcsB * g / (T(2) * cB);
   T guA = wA ? gvnA : T(0), gvA = wA ? gvtA : T(0);
   T guB = wB ? gvnB : T(0), gvB = wB ? gvtB : T(0);
   ghrA += gQnA * uA + gQtA * vA;

@@ -9,6 +9,8 @@
 //     two volume sums are float64 (per-workgroup partials in a fixed order,
 //     then one workgroup), so the rescale is deterministic and the corrected
 //     volume matches the physics volume to float32 rounding of each cell.
+// Also the batched staging of comparison/v2_staging.py (features of many
+// primitive states at once, and the depth correction of a batch).
 // Element-wise arithmetic follows the reference's float32 order (the build
 // uses -ffp-contract=off and IEEE division).
 
@@ -168,6 +170,47 @@ __global__ __launch_bounds__(kBlock) void k_corr_apply(Grid G, const float* H, c
   }
 }
 
+// Feature staging of a batch of primitive states (comparison/v2_staging.py:
+// 17-44): sample s, cell k, channel c of the (count, ny, nx, 3) NHWC state is
+// state[(s*N + k)*3 + c]; the output channel c of (s, k) goes to
+// out[s*ss + c*cs + k*ps].  blockIdx.y is the sample.
+__global__ __launch_bounds__(kBlock) void k_stage(size_t N, const float* __restrict__ state,
+                                                  const uint8_t* __restrict__ mask,
+                                                  const float* __restrict__ terrain,
+                                                  const float* __restrict__ rough, Norm nz,
+                                                  const float* __restrict__ rain_ch,
+                                                  float* __restrict__ out, long long ss,
+                                                  long long cs, long long ps) {
+  const size_t smp = blockIdx.y;
+  const float rain = rain_ch[smp];
+  const float* st = state + smp * N * 3;
+  float* o0 = out + (long long)smp * ss;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    const bool a = mask[k] != 0;
+    float* o = o0 + (long long)k * ps;
+    o[0] = a ? (st[3 * k] - nz.mean[0]) / nz.std[0] : 0.0f;
+    o[cs] = a ? (st[3 * k + 1] - nz.mean[1]) / nz.std[1] : 0.0f;
+    o[2 * cs] = a ? (st[3 * k + 2] - nz.mean[2]) / nz.std[2] : 0.0f;
+    o[3 * cs] = rain;
+    o[4 * cs] = terrain[k];
+    o[5 * cs] = rough[k];
+  }
+}
+
+// max(depth + residual[..., 0], 0) for a batch (v2_staging.py:47-52); residual
+// channel 0 of (s, k) is residual[s*ss + k*ps].
+__global__ __launch_bounds__(kBlock) void k_correct_depth(size_t N, const float* __restrict__ depth,
+                                                          const float* __restrict__ residual,
+                                                          long long ss, long long ps,
+                                                          float* __restrict__ out) {
+  const size_t smp = blockIdx.y;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock)
+    out[smp * N + k] = fmaxf(depth[smp * N + k] + residual[(long long)smp * ss + (long long)k * ps],
+                             0.0f);
+}
+
 int grid_blocks(size_t N) {
   return (int)std::max<size_t>(1, std::min<size_t>((N + kBlock - 1) / kBlock, 4096));
 }
@@ -240,6 +283,49 @@ int swe_apply_residual_correction(SweContext* c, SweState predicted, const float
     if (e != hipSuccess)
       return set_error(SWE_EHIP, std::string("residual correction volumes: ") + hipGetErrorString(e));
   }
+  return SWE_OK;
+}
+
+int swe_stage_features(int32_t count, int32_t ny, int32_t nx, const float* state_nhwc,
+                       const uint8_t* domain_mask, const float* terrain_std,
+                       const float* roughness_std, const float* state_mean, const float* state_std,
+                       const float* rain_channel, float* features, int64_t sample_stride,
+                       int64_t channel_stride, int64_t cell_stride, void* stream) {
+  if (count < 0 || count > 65535 || ny < 1 || nx < 1)
+    return set_error(SWE_EINVAL, "swe_stage_features: bad sizes");
+  if (count == 0) return SWE_OK;
+  if (!state_nhwc || !domain_mask || !terrain_std || !roughness_std || !state_mean || !state_std ||
+      !rain_channel || !features)
+    return set_error(SWE_EINVAL, "swe_stage_features: null argument");
+  const size_t N = (size_t)nx * (size_t)ny;
+  Norm nz;
+  for (int i = 0; i < 3; ++i) {
+    nz.mean[i] = state_mean[i];
+    nz.std[i] = state_std[i];
+  }
+  const int nb = std::min(grid_blocks(N), 1024);
+  hipLaunchKernelGGL(k_stage, dim3(nb, count), dim3(kBlock), 0, (hipStream_t)stream, N, state_nhwc,
+                     domain_mask, terrain_std, roughness_std, nz, rain_channel, features,
+                     (long long)sample_stride, (long long)channel_stride, (long long)cell_stride);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(SWE_EHIP, std::string("k_stage: ") + hipGetErrorString(e));
+  return SWE_OK;
+}
+
+int swe_correct_depth(int32_t count, int32_t ny, int32_t nx, const float* depth,
+                      const float* residual, int64_t sample_stride, int64_t cell_stride,
+                      float* out, void* stream) {
+  if (count < 0 || count > 65535 || ny < 1 || nx < 1)
+    return set_error(SWE_EINVAL, "swe_correct_depth: bad sizes");
+  if (count == 0) return SWE_OK;
+  if (!depth || !residual || !out) return set_error(SWE_EINVAL, "swe_correct_depth: null argument");
+  const size_t N = (size_t)nx * (size_t)ny;
+  const int nb = std::min(grid_blocks(N), 1024);
+  hipLaunchKernelGGL(k_correct_depth, dim3(nb, count), dim3(kBlock), 0, (hipStream_t)stream, N,
+                     depth, residual, (long long)sample_stride, (long long)cell_stride, out);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_error(SWE_EHIP, std::string("k_correct_depth: ") + hipGetErrorString(e));
   return SWE_OK;
 }
 

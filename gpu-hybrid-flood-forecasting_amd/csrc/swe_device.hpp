@@ -274,6 +274,20 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * per + min(x, rem) + k;
 }
 
+// Doubles move as two 32-bit halves.
+__device__ __forceinline__ double from_west(double x) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)from_west((int)(unsigned)b);
+  const unsigned hi = (unsigned)from_west((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double from_east(double x) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, 0x130, 0xF, 0xF, true);
+  const unsigned hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x130, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Index of the calling wavefront within its workgroup, as a wave-uniform
 // (scalar) value so that everything derived from it stays in SGPRs.
 __device__ __forceinline__ int wave_in_block() {

@@ -35,6 +35,10 @@ class SweState(ctypes.Structure):
     _fields_ = [("h", c_void_p), ("hu", c_void_p), ("hv", c_void_p)]
 
 
+class SweStateF64(ctypes.Structure):
+    _fields_ = [("h", c_void_p), ("hu", c_void_p), ("hv", c_void_p)]
+
+
 class SweParamsDesc(ctypes.Structure):
     _fields_ = [
         ("nx", c_int32),
@@ -135,6 +139,25 @@ SIGNATURES = {
         [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
          c_void_p, c_void_p],
     ),
+    "swe_stage_features": (
+        c_int32,
+        [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_float),
+         POINTER(c_float), c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p],
+    ),
+    "swe_correct_depth": (
+        c_int32,
+        [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p],
+    ),
+    "swe_adjoint_step_f64": (
+        c_int32,
+        [c_void_p, SweState, SweStateF64, SweStateF64, c_double, c_double, c_void_p, c_void_p,
+         c_void_p],
+    ),
+    "swe_adjoint_sweep_f64": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, POINTER(c_double), POINTER(c_double),
+         SweStateF64, SweStateF64, c_void_p, c_void_p, c_void_p],
+    ),
     "swe_run_steps": (
         c_int32, [c_void_p, SweState, c_int32, c_double, POINTER(SweRunInfo), c_void_p]
     ),
@@ -204,6 +227,13 @@ def stream_handle(device: torch.device | None = None) -> c_void_p:
 
 def state_struct(h: torch.Tensor, hu: torch.Tensor, hv: torch.Tensor) -> SweState:
     return SweState(h.data_ptr(), hu.data_ptr(), hv.data_ptr())
+
+
+def state_struct_f64(h: torch.Tensor, hu: torch.Tensor, hv: torch.Tensor) -> SweStateF64:
+    for t in (h, hu, hv):
+        if t.dtype != torch.float64:
+            raise TypeError("float64 cotangent tensors expected")
+    return SweStateF64(h.data_ptr(), hu.data_ptr(), hv.data_ptr())
 
 
 class Context:

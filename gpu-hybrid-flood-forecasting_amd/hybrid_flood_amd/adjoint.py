@@ -6,8 +6,9 @@ sweep replays steps with ``swe_step`` (bit-identical to the loop's steps,
 same kernels and the same float64 scalars), keeps one state every
 ``C = ceil(sqrt(K))`` steps, and walks the segments backwards: each segment is
 recomputed from its checkpoint into a C-state buffer, then reversed step by
-step with the hand-written adjoint kernel (``swe_adjoint_step``).  Memory is
-~2 sqrt(K) states; compute is ~3 forward steps + 1 reverse step per step.
+step with the hand-written adjoint kernel (``swe_adjoint_sweep``, or
+``swe_adjoint_sweep_f64`` for float64 cotangents).  Memory is ~2 sqrt(K)
+states; compute is ~3 forward steps + 1 reverse step per step.
 
 ``rollout_depths(manning_n, rain_multiplier, model)`` returns the depths at
 the model's output times and differentiates with respect to the per-cell
@@ -53,6 +54,7 @@ class RolloutModel:
     max_steps: int = 20_000
     variant: str = "fast"
     checkpoint_every: int = 0  # 0 = ceil(sqrt(K))
+    adjoint_precision: str = "f32"  # cotangent arithmetic of the reverse sweep: "f32" or "f64"
 
     def params_for(self, manning_n: torch.Tensor, rain_multiplier) -> SWEParams:
         mult = (float(rain_multiplier) * self.base_multiplier.double()).float().contiguous()
@@ -125,23 +127,57 @@ def rollout_forward(params: SWEParams, state0: SWEState, times, *, max_steps: in
     return res.states.h, res, tape
 
 
+def replay_depths(params: SWEParams, state0: SWEState, tape: Tape, *, variant: str) -> torch.Tensor:
+    """Depths (T, ny, nx) at the tape's output steps, replaying its frozen dt
+    schedule under ``params`` (e.g. perturbed parameters for a finite-difference
+    check of the adjoint, which differentiates the same frozen-schedule map)."""
+    ctx = context_for(params, variant)
+    lib = _lib.load()
+    s = _lib.stream_handle(state0.h.device)
+    st = _clone(state0)
+    out = [st.h.clone()]
+    pos = 0
+    for m in range(1, len(tape.out_steps)):
+        k = int(tape.out_steps[m])
+        if k > pos:
+            _lib.check(lib.swe_replay(ctx.handle, _lib.state_struct(*st), _dptr(tape.dts, pos),
+                                      _dptr(tape.ts, pos), k - pos, _lib.state_struct(*st), s),
+                       "swe_replay")
+            pos = k
+        out.append(st.h.clone())
+    return torch.stack(out)
+
+
 def _dptr(arr: np.ndarray, offset: int):
     """ctypes double* to arr[offset:] (arr must be a contiguous float64 array)."""
     return ctypes.cast(arr.ctypes.data + 8 * int(offset), ctypes.POINTER(ctypes.c_double))
 
 
-def rollout_backward(tape: Tape, grad_depths: torch.Tensor, *, variant: str):
-    """Reverse sweep; returns (grad_manning, grad_multiplier_field, lambda_0)."""
+def rollout_backward(tape: Tape, grad_depths: torch.Tensor, *, variant: str,
+                     precision: str = "f32"):
+    """Reverse sweep; returns (grad_manning, grad_multiplier_field, lambda_0).
+
+    ``precision="f32"`` runs the cotangents, the reverse arithmetic and the
+    sensitivities in float32; ``"f64"`` in float64 (gfx950 runs float64
+    add/mul/FMA at the float32 rate; the forward recompute stays the float32
+    kernel's either way).  On the tested rollouts the two agree to 1e-6.
+    """
+    if precision not in ("f64", "f32"):
+        raise ValueError("precision must be 'f64' or 'f32'")
+    f64 = precision == "f64"
     ctx = context_for(tape.params, variant)
     lib = _lib.load()
     h0 = tape.checkpoints[0].h
     dev = h0.device
     s = _lib.stream_handle(dev)
     N = h0.numel()
-    gn = torch.zeros_like(h0)
-    gm = torch.zeros_like(h0)
-    lam = SWEState(torch.zeros_like(h0), torch.zeros_like(h0), torch.zeros_like(h0))
-    tmp = SWEState(torch.empty_like(h0), torch.empty_like(h0), torch.empty_like(h0))
+    cdt = torch.float64 if f64 else torch.float32
+    gn = torch.zeros(h0.shape, dtype=cdt, device=dev)
+    gm = torch.zeros(h0.shape, dtype=cdt, device=dev)
+    lam = SWEState(*(torch.zeros(h0.shape, dtype=cdt, device=dev) for _ in range(3)))
+    tmp = SWEState(*(torch.empty(h0.shape, dtype=cdt, device=dev) for _ in range(3)))
+    lam_struct = _lib.state_struct_f64 if f64 else _lib.state_struct
+    sweep = lib.swe_adjoint_sweep_f64 if f64 else lib.swe_adjoint_sweep
     outs_at: dict[int, list[int]] = {}
     for m, k in enumerate(tape.out_steps):
         outs_at.setdefault(int(k), []).append(m)
@@ -174,10 +210,10 @@ def rollout_backward(tape: Tape, grad_depths: torch.Tensor, *, variant: str):
         for k in stops:
             if k < pos:
                 j = k - start
-                _lib.check(lib.swe_adjoint_sweep(
+                _lib.check(sweep(
                     ctx.handle, cube[0, j].data_ptr(), cube[1, j].data_ptr(), cube[2, j].data_ptr(),
-                    pos - k, _dptr(dts, k), _dptr(ts, k), _lib.state_struct(*lam),
-                    _lib.state_struct(*tmp), gn.data_ptr(), gm.data_ptr(), s), "swe_adjoint_sweep")
+                    pos - k, _dptr(dts, k), _dptr(ts, k), lam_struct(*lam), lam_struct(*tmp),
+                    gn.data_ptr(), gm.data_ptr(), s), "swe_adjoint_sweep")
                 pos = k
                 inject(k)
     return gn, gm, lam
@@ -199,9 +235,10 @@ class _RolloutFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_depths):
         model: RolloutModel = ctx.model
-        gn, gm, _ = rollout_backward(ctx.tape, grad_depths.contiguous().float(), variant=model.variant)
+        gn, gm, _ = rollout_backward(ctx.tape, grad_depths.contiguous().float(), variant=model.variant,
+                                     precision=model.adjoint_precision)
         g_alpha = (gm.double() * model.base_multiplier.double()).sum()
-        return gn, g_alpha.to(ctx.mult_dtype), None
+        return gn.float(), g_alpha.to(ctx.mult_dtype), None
 
 
 def rollout_depths(manning_n: torch.Tensor, rain_multiplier: torch.Tensor, model: RolloutModel):

@@ -60,6 +60,13 @@ typedef struct SweState {
   float* hv;
 } SweState;
 
+/* A cotangent of the state held in float64 (the adjoint's *_f64 entries). */
+typedef struct SweStateF64 {
+  double* h;
+  double* hu;
+  double* hv;
+} SweStateF64;
+
 /* Static fields and controls (jax_solver/numerics.py:29-45, built by
  * ShallowWaterSolver.__init__ at jax_solver/shallow_water_2d.py:213-253).
  * Device pointers must stay valid for the life of the context; the rainfall
@@ -238,6 +245,20 @@ int swe_adjoint_sweep(SweContext* ctx, const float* states_h, const float* state
                       SweState lam, SweState lam_tmp, float* grad_manning, float* grad_mult,
                       void* stream);
 
+/* swe_adjoint_step / swe_adjoint_sweep with the cotangents, the reverse
+ * arithmetic and the sensitivities in float64 (the forward recompute stays
+ * the float32 kernel's, branch for branch) for callers that want the
+ * reverse sweep free of float32 rounding; gfx950 runs float64 add/mul/FMA at
+ * the float32 rate.  No reference counterpart (the reference has no
+ * adjoint). */
+int swe_adjoint_step_f64(SweContext* ctx, SweState state, SweStateF64 lam_next,
+                         SweStateF64 lam_out, double dt, double time_s, double* grad_manning,
+                         double* grad_mult, void* stream);
+int swe_adjoint_sweep_f64(SweContext* ctx, const float* states_h, const float* states_hu,
+                          const float* states_hv, int32_t n, const double* dts, const double* ts,
+                          SweStateF64 lam, SweStateF64 lam_tmp, double* grad_manning,
+                          double* grad_mult, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched ensemble (no counterpart in the reference, which runs one domain;
  * it runs many independent rainfall/roughness scenarios of the same step).
@@ -309,6 +330,26 @@ int swe_apply_residual_correction(SweContext* ctx, SweState predicted, const flo
                                   int64_t channel_stride, int64_t cell_stride, float relaxation,
                                   const float* clip_limits, SweState corrected, double* volumes,
                                   void* stream);
+
+/* Feature staging of `count` primitive states (comparison/v2_staging.py:17-44):
+ * state_nhwc is device float32 [count][ny][nx][3] (depth, u, v); output
+ * channels (standardised state, masked; rain_channel[s] (device float32
+ * [count], the sample's rain / scale); terrain_std; roughness_std) of sample
+ * s, cell k, channel c go to features[s*sample_stride + c*channel_stride +
+ * k*cell_stride].  No context needed.  Asynchronous. */
+int swe_stage_features(int32_t count, int32_t ny, int32_t nx, const float* state_nhwc,
+                       const uint8_t* domain_mask, const float* terrain_std,
+                       const float* roughness_std, const float* state_mean, const float* state_std,
+                       const float* rain_channel, float* features, int64_t sample_stride,
+                       int64_t channel_stride, int64_t cell_stride, void* stream);
+
+/* out[s][k] = max(depth[s][k] + residual channel 0, 0) for `count` samples
+ * (comparison/v2_staging.py:47-52); the residual of (s, k) is read at
+ * residual[s*sample_stride + k*cell_stride], so an NCHW network output is
+ * used in place.  Asynchronous. */
+int swe_correct_depth(int32_t count, int32_t ny, int32_t nx, const float* depth,
+                      const float* residual, int64_t sample_stride, int64_t cell_stride,
+                      float* out, void* stream);
 
 /* ---- Output schema (jax_solver/shallow_water_2d.py:309-350,
  *      structured_io.py:160-211) ------------------------------------------ */

@@ -3,7 +3,8 @@
 TEST INFRASTRUCTURE ONLY: the checker for ``hybrid_flood_amd.coupled_forecast``.
 Only ``tests/`` may import it; the product path never calls it.
 
-Restates ``src/hybrid_flood/hybrid/coupled_forecast.py`` in NumPy float32:
+Restates ``src/hybrid_flood/hybrid/coupled_forecast.py`` (and the staging of
+``src/hybrid_flood/comparison/v2_staging.py``) in NumPy float32:
 
 * ``network_features``          -- coupled_forecast.py:53-79;
 * ``apply_residual_correction`` -- coupled_forecast.py:82-120, with the two
@@ -11,12 +12,15 @@ Restates ``src/hybrid_flood/hybrid/coupled_forecast.py`` in NumPy float32:
   depth rounded once from float64, as the HIP kernel does;
 * ``run_coupled_forecast``      -- coupled_forecast.py:164-231, with the
   physics advance of ``swe_oracle.advance_physics_to_time`` (float64 clock)
-  and any NumPy callable as the network (NHWC in, NHWC out, batch 1).
+  and any NumPy callable as the network (NHWC in, NHWC out, batch 1);
+* ``stage_features`` / ``correct_depth`` -- the batched staging of
+  comparison/v2_staging.py:17-52.
 
 Pinned in ``tests/test_oracle_coupled.py`` by closed forms: zero residual
 keeps the masked depth, the corrected volume equals the physics volume for
 any residual, positivity, clipping equals pre-clipped input, uniform-state
-features, the piecewise-linear rain channel.
+features, the piecewise-linear rain channel, staged channels of known
+inputs.
 """
 
 from __future__ import annotations
@@ -120,3 +124,26 @@ def run_coupled_forecast(initial, params, model, terrain, roughness, state_mean,
         outs.append(state)
     stack = so.OracleState(*(np.stack([getattr(s, f) for s in outs]) for f in ("h", "hu", "hv")))
     return stack, times, total
+
+
+def stage_features(state_t, rainfall_t_mm_hr, terrain, roughness, domain_mask, state_mean,
+                   state_std, rainfall_scale_mm_hr):
+    """(count, ny, nx, 6) float32 features of primitive states (comparison/v2_staging.py:17-44)."""
+    st = np.asarray(state_t, np.float32)
+    mean = np.asarray(state_mean, np.float32)
+    std = np.asarray(state_std, np.float32)
+    dyn = ((st - mean) / std).astype(np.float32)
+    dyn = np.where(np.asarray(domain_mask, bool)[None, ..., None], dyn, np.float32(0))
+    count, ny, nx, _ = st.shape
+    rain = (np.asarray(rainfall_t_mm_hr, np.float32) / np.float32(rainfall_scale_mm_hr)).astype(np.float32)
+    return np.concatenate((
+        dyn, np.broadcast_to(rain[:, None, None, None], (count, ny, nx, 1)),
+        np.broadcast_to(np.asarray(terrain, np.float32)[None, ..., None], (count, ny, nx, 1)),
+        np.broadcast_to(np.asarray(roughness, np.float32)[None, ..., None], (count, ny, nx, 1))),
+        axis=-1).astype(np.float32)
+
+
+def correct_depth(raw_depth, residual_nhwc):
+    """max(depth + residual[..., 0], 0) per sample (comparison/v2_staging.py:47-52)."""
+    d = np.asarray(raw_depth, np.float32)
+    return np.maximum(d + np.asarray(residual_nhwc, np.float32)[..., 0], np.float32(0))

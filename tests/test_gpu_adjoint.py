@@ -229,3 +229,109 @@ def test_adjoint_step_matches_cpu_adjoint_oracle(cuda, shape, sides, variant):
         tol = 1e-3 * np.abs(want) + 1e-4 * max(np.max(np.abs(want)), 1e-30)
         bad = int(np.sum(np.abs(got - want) > tol))
         assert bad <= max(1, want.size // 1000), f"{bad} cells off"
+
+
+@pytest.mark.parametrize("variant", ["fast", "reference"])
+def test_long_dam_break_gradient_matches_frozen_fd(cuda, variant):
+    """256^2 urban dam break, ~2000 steps: a roof-drainage flow with wet cells
+    whose water level sits within the dry tolerance of a neighbour's bed.  The
+    rollout gradient along a smooth Manning-n direction must match central
+    differences of the same frozen-schedule map (replayed on the GPU, eps 1e-2
+    and 1e-1 of a 0.01-amplitude direction, which agree to 0.3 %) to 1 %."""
+    import torch
+
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd import synthetic as sy
+    from hybrid_flood_amd.adjoint import (RolloutModel, replay_depths, rollout_depths,
+                                          rollout_forward)
+
+    n, steps = 256, 2000
+    case = sy.urban_case(n, dx=2.0, seed=7)
+    h = torch.as_tensor(sy.dam_break_state(n), device=cuda)
+    state0 = nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h))
+    p0 = nm.make_params(case.bed, case.manning_n, case.domain_mask, boundary_types=case.boundary_types,
+                        rainfall_times_s=case.rain_times_s, rainfall_rates_m_s=case.rain_rates_m_s,
+                        dx=case.dx, device=cuda)
+    probe = nm.SWEState(*(x.clone() for x in state0))
+    horizon = nm.run_steps(probe, p0, steps, variant=variant, diagnostics=True)["final_time_s"]
+    times = np.linspace(0.0, horizon, 11).astype(np.float32)
+    model = RolloutModel(bed=p0.bed, domain_mask=torch.ones((n, n), dtype=torch.bool, device=cuda),
+                         base_multiplier=torch.ones_like(h), h0=state0, output_times_s=times,
+                         boundary_types=case.boundary_types, rainfall_times_s=case.rain_times_s,
+                         rainfall_rates_m_s=case.rain_rates_m_s, dx=case.dx, max_steps=4 * steps,
+                         variant=variant)
+    hidden = torch.as_tensor(sy.smooth_field(n, np.random.default_rng(11), 0.01, 0.06), device=cuda)
+    with torch.no_grad():
+        target = rollout_depths(hidden, torch.tensor(1.3), model).clone()
+
+    def rmse(d):
+        return torch.sqrt(((d[1:].double() - target[1:].double()) ** 2).mean())
+
+    n0 = torch.as_tensor(case.manning_n, device=cuda)
+    n_var = n0.clone().requires_grad_(True)
+    rmse(rollout_depths(n_var, torch.tensor(1.0, dtype=torch.float64), model)).backward()
+    dn = torch.as_tensor(sy.smooth_field(n, np.random.default_rng(3), -0.01, 0.01), device=cuda).double()
+    adj = float((n_var.grad.double() * dn).sum())
+    _, _, tape = rollout_forward(model.params_for(n0, 1.0), state0, times, max_steps=4 * steps,
+                                 variant=variant)
+
+    def L(nf):
+        return float(rmse(replay_depths(model.params_for(nf.float(), 1.0), state0, tape, variant=variant)))
+
+    fd = {e: (L(n0.double() + e * dn) - L(n0.double() - e * dn)) / (2 * e) for e in (1e-1, 1e-2)}
+    assert fd[1e-2] == pytest.approx(fd[1e-1], rel=3e-3)
+    assert adj == pytest.approx(fd[1e-2], rel=1e-2)
+
+
+@pytest.mark.parametrize("variant", ["fast", "reference"])
+def test_adjoint_step_f64_matches_cpu_adjoint_oracle(cuda, variant):
+    """The float64-cotangent step against the float64 CPU assembly: only the
+    float32 forward recompute separates them (1e-5 relative + 1e-6 of the scale;
+    <= 0.1 % of cells may sit on a kink the float32 forward takes differently)."""
+    import torch
+
+    from hybrid_flood_amd import _lib
+    from hybrid_flood_amd import numerics as nm
+    from oracle import swe_adjoint_oracle as ao
+
+    shape = (37, 70)
+    case = random_case(52, shape, open_sides=(0, 1, 1, 0), speed=0.8, dry_frac=0.15, mask_frac=0.08)
+    p, st = _gpu(case, cuda)
+    ctx = nm.context_for(p, variant)
+    rng = np.random.default_rng(6)
+    w = [np.where(case["domain_mask"], rng.normal(size=shape), 0) for _ in range(3)]
+    dt, t = 0.04, 75.0
+    lam = nm.SWEState(*(torch.zeros(shape, dtype=torch.float64, device=cuda) for _ in range(3)))
+    gn = torch.zeros(shape, dtype=torch.float64, device=cuda)
+    gm = torch.zeros(shape, dtype=torch.float64, device=cuda)
+    _lib.check(_lib.load().swe_adjoint_step_f64(
+        ctx.handle, _lib.state_struct(*st),
+        _lib.state_struct_f64(*(torch.as_tensor(x, device=cuda) for x in w)),
+        _lib.state_struct_f64(*lam), dt, t, gn.data_ptr(), gm.data_ptr(), _lib.stream_handle()),
+        "swe_adjoint_step_f64")
+    want_lam, want_gn, want_gm = ao.step_vjp((case["h"], case["hu"], case["hv"]), _o64(case), dt, t, w)
+    for got, want in zip(list(lam) + [gn, gm], list(want_lam) + [want_gn, want_gm]):
+        got = got.cpu().numpy()
+        tol = 1e-5 * np.abs(want) + 1e-6 * max(np.max(np.abs(want)), 1e-30)
+        bad = int(np.sum(np.abs(got - want) > tol))
+        assert bad <= max(1, want.size // 1000), f"{bad} cells off"
+
+
+def test_f64_and_f32_sweeps_agree(cuda):
+    import torch
+
+    from hybrid_flood_amd.adjoint import depth_rmse, rollout_depths
+
+    case, model = _rollout_setup(cuda)
+    target = torch.zeros((len(model.output_times_s),) + case["h"].shape, device=cuda)
+    grads = {}
+    for prec in ("f32", "f64"):
+        model.adjoint_precision = prec
+        n0 = torch.as_tensor(case["manning_n"], device=cuda).clone().requires_grad_(True)
+        alpha = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+        depth_rmse(rollout_depths(n0, alpha, model), target).backward()
+        grads[prec] = (n0.grad.double().cpu().numpy(), float(alpha.grad))
+    model.adjoint_precision = "f32"
+    scale = np.abs(grads["f64"][0]).max()
+    np.testing.assert_allclose(grads["f32"][0], grads["f64"][0], rtol=1e-3, atol=1e-4 * scale)
+    assert grads["f32"][1] == pytest.approx(grads["f64"][1], rel=1e-4)

@@ -215,3 +215,31 @@ def test_result_to_dataset_matches_oracle(cuda, tmp_path):
     back = load_npz(ds.to_npz(tmp_path / "out.npz"))
     assert np.array_equal(back["depth"], want["depth"], equal_nan=True)
     assert back["__meta__"]["var_attrs"]["velocity"]["units"] == "m s-1"
+
+
+# ---------------------------------------------------------------------------
+# Feature staging (comparison/v2_staging.py)
+# ---------------------------------------------------------------------------
+
+def test_stage_features_and_correct_depth_bit_exact(cuda):
+    import torch
+
+    from hybrid_flood_amd.staging import correct_depth, stage_features
+    from oracle import coupled_oracle as co
+
+    rng = np.random.default_rng(9)
+    count, ny, nx = 4, 19, 45
+    st = rng.normal(0, 1, (count, ny, nx, 3)).astype(np.float32)
+    mask = rng.uniform(size=(ny, nx)) > 0.2
+    terrain, rough = rng.normal(size=(2, ny, nx)).astype(np.float32)
+    rain = rng.uniform(0, 40, count).astype(np.float32)
+    got = stage_features(torch.as_tensor(st, device=cuda), rain, terrain, rough, mask,
+                         [0.1, -0.2, 0.05], [0.3, 0.7, 0.9], 25.0)
+    assert tuple(got.shape) == (count, 6, ny, nx)
+    want = co.stage_features(st, rain, terrain, rough, mask, [0.1, -0.2, 0.05], [0.3, 0.7, 0.9], 25.0)
+    assert np.array_equal(got.permute(0, 2, 3, 1).cpu().numpy(), want)
+    pred = torch.as_tensor(rng.normal(0, 0.5, (count, 3, ny, nx)).astype(np.float32), device=cuda)
+    depth = np.abs(rng.normal(0, 0.4, (count, ny, nx))).astype(np.float32)
+    out = correct_depth(depth, pred)
+    assert np.array_equal(out.cpu().numpy(),
+                          co.correct_depth(depth, pred.permute(0, 2, 3, 1).cpu().numpy()))

@@ -182,3 +182,20 @@ def test_coupled_loop_zero_model_is_physics_for_depth():
     ref, rsteps, _ = o.advance_physics_to_time(st, p32, 0.0, 5.0, max_substeps=64)
     np.testing.assert_array_equal(stack.h[1], ref.h)
     assert steps >= rsteps > 0
+
+
+def test_stage_features_closed_form():
+    co = _co()
+    rng = np.random.default_rng(0)
+    st = rng.normal(size=(3, 5, 7, 3)).astype(np.float32)
+    mask = rng.uniform(size=(5, 7)) > 0.3
+    f = co.stage_features(st, [0.0, 12.0, 30.0], np.ones((5, 7)), -np.ones((5, 7)), mask,
+                          [1.0, 0.0, -1.0], [2.0, 1.0, 4.0], 6.0)
+    assert f.shape == (3, 5, 7, 6)
+    np.testing.assert_allclose(f[:, mask, 0], (st[:, mask, 0] - 1.0) / 2.0, rtol=1e-7)
+    np.testing.assert_allclose(f[:, mask, 2], (st[:, mask, 2] + 1.0) / 4.0, rtol=1e-7)
+    assert np.all(f[:, ~mask, :3] == 0)
+    np.testing.assert_array_equal(f[:, 2, 3, 3], [0.0, 2.0, 5.0])
+    assert np.all(f[..., 4] == 1) and np.all(f[..., 5] == -1)
+    d = co.correct_depth(np.array([[0.5, 0.1]], np.float32), np.array([[[0.25, 9], [-0.3, 9]]], np.float32))
+    np.testing.assert_array_equal(d, [[0.75, 0.0]])
