@@ -195,10 +195,6 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
 using namespace swe;
 
 namespace {
-constexpr int kAdjRows = 16;
-}
-
-namespace {
 template <class T>
 int launch_adjoint(SweContext* c, const float* h, const float* hu, const float* hv, const T* lh,
                    const T* lhu, const T* lhv, T* oh, T* ohu, T* ohv, double dt, double time_s,
@@ -289,17 +285,18 @@ int launch_adjoint(SweContext* c, const float* h, const float* hu, const float* 
   const double rate = context_rate(c, time_s);
   const float dt_f = (float)dt, dtrate_f = (float)(dt * rate);
   const float dtg_f = (float)(dt * context_gravity(c));
+  const int rows = context_adjoint_rows(c);
   const int segs_x = (G.nx + kAdjCols - 1) / kAdjCols;
-  const int n_waves = segs_x * ((G.ny + kAdjRows - 1) / kAdjRows);
+  const int n_waves = segs_x * ((G.ny + rows - 1) / rows);
   const int blocks = (n_waves + 3) / 4;
   if (context_variant(c) == SWE_VARIANT_REFERENCE)
     hipLaunchKernelGGL((k_adjoint_step<RefMath, T>), dim3(blocks), dim3(256), 0, s, G, h, hu, hv,
                        lh, lhu, lhv, oh, ohu, ohv, grad_manning, grad_mult, dt_f, dtrate_f, dtg_f,
-                       kAdjRows, segs_x, n_waves);
+                       rows, segs_x, n_waves);
   else
     hipLaunchKernelGGL((k_adjoint_step<FastMath, T>), dim3(blocks), dim3(256), 0, s, G, h, hu, hv,
                        lh, lhu, lhv, oh, ohu, ohv, grad_manning, grad_mult, dt_f, dtrate_f, dtg_f,
-                       kAdjRows, segs_x, n_waves);
+                       rows, segs_x, n_waves);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(SWE_EHIP, std::string("swe_adjoint_step: ") + hipGetErrorString(e));
   return SWE_OK;

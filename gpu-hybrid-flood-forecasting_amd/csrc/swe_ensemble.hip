@@ -274,23 +274,55 @@ int launch_chunks(SweEnsemble* e, int launches, hipStream_t s) {
 }
 
 // Launch per-member init reductions (CFL, volume) that prime each member's control.
+// Start of a run, first half: many workgroups per member copy the caller's
+// state into the member's work buffer A and write per-workgroup partials of
+// the CFL maximum, the FP64 volume and the non-finite flag (slots
+// wave_base .. wave_base + gridDim.x - 1 of the partials, gridDim.x <= n_waves).
 template <class MM>
-__global__ __launch_bounds__(256) void k_ens_init(const Member* __restrict__ mem, int m,
-                                                 EnsPartials P) {
-  // One workgroup per member: a grid-stride pass over the member's cells.
+__global__ __launch_bounds__(256) void k_ens_load(const Member* __restrict__ mem, int m,
+                                                  const float* __restrict__ h,
+                                                  const float* __restrict__ hu,
+                                                  const float* __restrict__ hv, EnsPartials P) {
   const Member& mb = mem[m];
   const Grid& G = mb.G;
   const size_t N = (size_t)G.nx * (size_t)G.ny;
-  const float* H = mb.buf[0];
+  float* A = mb.buf[0];
   float inv = 0.0f;
   double vol = 0.0;
   int nf = 0;
-  for (size_t k = threadIdx.x; k < N; k += blockDim.x) {
-    const float h = H[k], hu = H[N + k], hv = H[2 * N + k];
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const float a0 = h[k], a1 = hu[k], a2 = hv[k];
+    A[k] = a0;
+    A[N + k] = a1;
+    A[2 * N + k] = a2;
     const int a = G.mask[k] != 0;
-    inv = fmaxf(inv, cfl_inverse<MM>(G, h, hu, hv, a));
-    vol += a ? (double)h : 0.0;
-    nf |= !(isfinite(h) && isfinite(hu) && isfinite(hv));
+    inv = fmaxf(inv, cfl_inverse<MM>(G, a0, a1, a2, a));
+    vol += a ? (double)a0 : 0.0;
+    nf |= !(isfinite(a0) && isfinite(a1) && isfinite(a2));
+  }
+  const Totals tot = block_reduce(inv, vol, 0.0, 0.0, nf);
+  if (threadIdx.x == 0) {
+    const int q = mb.wave_base + blockIdx.x;
+    P.inv[q] = tot.inv;
+    P.vol[q] = tot.vol;
+    P.nf[q] = tot.nf;
+  }
+}
+
+// Second half, one workgroup: reduce the `parts` partials and plan the
+// member's first step.
+__global__ __launch_bounds__(256) void k_ens_init(const Member* __restrict__ mem, int m,
+                                                  int parts, EnsPartials P) {
+  const Member& mb = mem[m];
+  float inv = 0.0f;
+  double vol = 0.0;
+  int nf = 0;
+  for (int k = threadIdx.x; k < parts; k += blockDim.x) {
+    const int q = mb.wave_base + k;
+    inv = fmaxf(inv, P.inv[q]);
+    vol += P.vol[q];
+    nf |= P.nf[q];
   }
   const Totals tot = block_reduce(inv, vol, 0.0, 0.0, nf);
   const double t0 = mb.L.times[0];
@@ -319,23 +351,16 @@ __global__ __launch_bounds__(256) void k_ens_init(const Member* __restrict__ mem
       plan_next_step(ctl, mb.L, tot.inv, t0, 1, s_j);
     }
   }
-  (void)P;
 }
 
+// Copy a member's final work state [h | hu | hv] out to the caller's arrays.
 __global__ void k_ens_copy(const float* __restrict__ src, float* __restrict__ h,
-                           float* __restrict__ hu, float* __restrict__ hv, size_t N, int to_dst) {
+                           float* __restrict__ hu, float* __restrict__ hv, size_t N) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < N;
        k += (size_t)gridDim.x * blockDim.x) {
-    if (to_dst) {
-      h[k] = src[k];
-      hu[k] = src[N + k];
-      hv[k] = src[2 * N + k];
-    } else {
-      float* d = const_cast<float*>(src);
-      d[k] = h[k];
-      d[N + k] = hu[k];
-      d[2 * N + k] = hv[k];
-    }
+    h[k] = src[k];
+    hu[k] = src[N + k];
+    hv[k] = src[2 * N + k];
   }
 }
 
@@ -504,12 +529,17 @@ int swe_ensemble_begin(SweEnsemble* e, double t_end, int32_t max_steps, void* st
   for (int m = 0; m < n; ++m) {
     const SweState& st = e->desc[m].state;
     const size_t N = (size_t)e->host[m].G.nx * e->host[m].G.ny;
-    const int nb = (int)std::min<size_t>((N + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_ens_copy, dim3(nb), dim3(256), 0, s, e->host[m].buf[0], st.h, st.hu, st.hv, N, 0);
+    // Workgroups of the load pass: ~8 cells per thread, at most one per
+    // partial slot the member owns.
+    const int nb = (int)std::max<size_t>(
+        1, std::min<size_t>((N + 2047) / 2048, (size_t)e->act[m].n_waves));
     if (e->variant == SWE_VARIANT_REFERENCE)
-      hipLaunchKernelGGL(k_ens_init<RefMath>, dim3(1), dim3(256), 0, s, e->d_mem, m, e->P);
+      hipLaunchKernelGGL(k_ens_load<RefMath>, dim3(nb), dim3(256), 0, s, e->d_mem, m, st.h, st.hu,
+                         st.hv, e->P);
     else
-      hipLaunchKernelGGL(k_ens_init<FastMath>, dim3(1), dim3(256), 0, s, e->d_mem, m, e->P);
+      hipLaunchKernelGGL(k_ens_load<FastMath>, dim3(nb), dim3(256), 0, s, e->d_mem, m, st.h, st.hu,
+                         st.hv, e->P);
+    hipLaunchKernelGGL(k_ens_init, dim3(1), dim3(256), 0, s, e->d_mem, m, nb, e->P);
   }
   ENS_TRY(hipGetLastError());
   e->steps_at_begin.assign(n, 0);
@@ -564,7 +594,7 @@ int swe_ensemble_end(SweEnsemble* e, SweRunInfo* infos, void* stream) {
     const size_t N = (size_t)e->host[m].G.nx * e->host[m].G.ny;
     const int nb = (int)std::min<size_t>((N + 255) / 256, 4096);
     hipLaunchKernelGGL(k_ens_copy, dim3(nb), dim3(256), 0, s, e->host[m].buf[k.steps & 1], st.h,
-                       st.hu, st.hv, N, 1);
+                       st.hu, st.hv, N);
     if (infos) {
       SweRunInfo& I = infos[m];
       I.outputs_written = k.out_idx;

@@ -13,6 +13,8 @@ int context_variant(const SweContext* c);
 double context_gravity(const SweContext* c);
 double context_rate(const SweContext* c, double t);
 int context_reduce_blocks(const SweContext* c);
+// Strip height of the adjoint kernel.
+int context_adjoint_rows(const SweContext* c);
 // Device scratch of at least n_doubles doubles owned by the context (grown on
 // demand, kept until swe_destroy); nullptr if the allocation failed.
 double* context_scratch(SweContext* c, size_t n_doubles);

@@ -291,6 +291,7 @@ struct SweContext {
   double* aux = nullptr;  // scratch of the coupled-forecast kernels
   size_t aux_cap = 0;
   int lds_per_cu = 0;        // bytes of LDS per compute unit
+  int adjoint_rows = 0;      // strip height of the adjoint kernel (0 = default_rows)
   unsigned step_lds = 0;     // dynamic LDS reserved per step workgroup (occupancy cap)
 
   size_t cells() const { return (size_t)G.nx * (size_t)G.ny; }
@@ -446,7 +447,12 @@ int launch_loop_chunk(SweContext* c, int count, const LoopConst& L, float* ch, f
                                     (const void*)(intptr_t)c->graph_chunk, c->d_times,
                                     (const void*)(intptr_t)L.n_times,
                                     (const void*)(intptr_t)L.max_steps,
-                                    (const void*)(intptr_t)c->split_finish};
+                                    (const void*)(intptr_t)c->split_finish,
+                                    (const void*)(intptr_t)c->step_lds};
+    // The loop constants are baked into the captured kernel arguments.
+    uintptr_t words[(sizeof(LoopConst) + sizeof(uintptr_t) - 1) / sizeof(uintptr_t)] = {};
+    std::memcpy(words, &L, sizeof(LoopConst));
+    for (uintptr_t w : words) key.push_back((const void*)w);
     if (!c->graph || key != c->graph_key) {
       if (c->graph) {
         SWE_HIP_TRY(hipGraphExecDestroy(c->graph));
@@ -579,6 +585,9 @@ int swe::context_variant(const SweContext* c) { return c->variant; }
 double swe::context_gravity(const SweContext* c) { return c->gravity; }
 double swe::context_rate(const SweContext* c, double t) { return host_interp(c, t); }
 int swe::context_reduce_blocks(const SweContext* c) { return c->reduce_blocks(); }
+int swe::context_adjoint_rows(const SweContext* c) {
+  return c->adjoint_rows > 0 ? c->adjoint_rows : default_rows(c->G.nx, c->G.ny);
+}
 double* swe::context_scratch(SweContext* c, size_t n_doubles) {
   if (n_doubles > c->aux_cap) {
     if (c->aux && hipFree(c->aux) != hipSuccess) return nullptr;
@@ -691,6 +700,7 @@ int swe_create(const SweParamsDesc* p, int32_t variant, SweContext** out) {
     return cleanup(fail(SWE_EHIP, "swe_create: clear control"));
   if (hipHostMalloc(&c->h_ctl, 2 * sizeof(LoopControl), hipHostMallocDefault) != hipSuccess)
     return cleanup(fail(SWE_EHIP, "swe_create: hipHostMalloc control"));
+  c->rows = default_rows(G.nx, G.ny);
   int rc = ensure_partials(c);
   if (rc) return cleanup(rc);
   // Sum of the rain multiplier over active cells, for the FP64 rain volume.
@@ -734,8 +744,15 @@ int swe_set_variant(SweContext* c, int32_t variant) {
 int swe_set_tile_rows(SweContext* c, int32_t rows) {
   if (!c) return fail(SWE_EINVAL, "swe_set_tile_rows: null context");
   if (rows < 0 || rows > 4096) return fail(SWE_EINVAL, "swe_set_tile_rows: rows must be in [0, 4096]");
-  c->rows = rows == 0 ? kDefaultRows : rows;
+  c->rows = rows == 0 ? default_rows(c->G.nx, c->G.ny) : rows;
   return ensure_partials(c);
+}
+
+int swe_set_adjoint_rows(SweContext* c, int32_t rows) {
+  if (!c) return fail(SWE_EINVAL, "swe_set_adjoint_rows: null context");
+  if (rows < 0 || rows > 4096) return fail(SWE_EINVAL, "swe_set_adjoint_rows: rows must be in [0, 4096]");
+  c->adjoint_rows = rows;
+  return SWE_OK;
 }
 
 int swe_set_waves_per_cu(SweContext* c, int32_t waves) {

@@ -94,6 +94,7 @@ SIGNATURES = {
     "swe_set_variant": (c_int32, [c_void_p, c_int32]),
     "swe_set_tile_rows": (c_int32, [c_void_p, c_int32]),
     "swe_set_waves_per_cu": (c_int32, [c_void_p, c_int32]),
+    "swe_set_adjoint_rows": (c_int32, [c_void_p, c_int32]),
     "swe_set_graph_chunk": (c_int32, [c_void_p, c_int32]),
     "swe_set_dt_log": (c_int32, [c_void_p, c_void_p, c_int32]),
     "swe_set_loop_finish": (c_int32, [c_void_p, c_int32]),
@@ -284,6 +285,9 @@ class Context:
 
     def set_waves_per_cu(self, waves: int) -> None:
         check(load().swe_set_waves_per_cu(self.handle, int(waves)), "swe_set_waves_per_cu")
+
+    def set_adjoint_rows(self, rows: int) -> None:
+        check(load().swe_set_adjoint_rows(self.handle, int(rows)), "swe_set_adjoint_rows")
 
     def set_loop_finish(self, split: int) -> None:
         check(load().swe_set_loop_finish(self.handle, int(split)), "swe_set_loop_finish")

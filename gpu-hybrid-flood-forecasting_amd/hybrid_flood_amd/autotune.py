@@ -25,9 +25,11 @@ import torch
 from . import _lib
 from .numerics import SWEParams, SWEState, context_for
 
-CANDIDATE_ROWS = (8, 12, 16, 24, 32, 48, 64)
+# Heights between the round numbers let the last round of wavefronts fill the
+# device better (the wavefront count is segs_x * ceil(ny / rows)).
+CANDIDATE_ROWS = (4, 6, 8, 12, 14, 16, 17, 20, 21, 24, 32, 48, 64)
 # Resident wavefronts per compute unit: 0 = as many as registers allow.
-CANDIDATE_WAVES = (0, 20, 16)
+CANDIDATE_WAVES = (0, 20, 16, 12)
 
 
 def cache_path() -> Path:

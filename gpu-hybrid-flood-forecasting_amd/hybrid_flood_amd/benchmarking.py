@@ -29,7 +29,7 @@ def benchmark_compiled_steps(state: SWEState, params: SWEParams, *, iterations: 
         raise ValueError("Benchmark iterations must be positive.")
     records: list[dict[str, Any]] = [
         {"platform": "cpu", "available": False,
-         "reason": "The HIP step has no CPU backend; the CPU oracle lives in tests/oracle."}
+         "reason": "The HIP step has no CPU backend."}
     ]
     lib = _lib.load()
     ctx = context_for(params, variant)

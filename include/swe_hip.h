@@ -113,13 +113,18 @@ const char* swe_last_error(void);
 int swe_create(const SweParamsDesc* params, int32_t variant, SweContext** out);
 int swe_destroy(SweContext* ctx);
 int swe_set_variant(SweContext* ctx, int32_t variant);
-/* Rows per wavefront strip of the fused step (tuning knob; 0 = default). */
+/* Rows per wavefront strip of the fused step (tuning knob; 0 = default:
+ * about sqrt(nx * ny) / 170, between 4 and 16). */
 int swe_set_tile_rows(SweContext* ctx, int32_t rows);
 /* Cap the fused step at `waves` resident wavefronts per compute unit by
  * reserving LDS per workgroup (tuning knob: fewer concurrent row streams can
  * raise HBM efficiency).  0 = no cap; else a multiple of 4, >= 8.  No
  * reference counterpart (the reference has no tiling). */
 int swe_set_waves_per_cu(SweContext* ctx, int32_t waves);
+/* Rows per wavefront strip of the adjoint step (tuning knob; 0 = default,
+ * the same size rule as swe_set_tile_rows).  No
+ * reference counterpart (the reference differentiates with jax.grad). */
+int swe_set_adjoint_rows(SweContext* ctx, int32_t rows);
 /* How the device loop closes a step (tuning knob): 1 (default) = every
  * workgroup stores its CFL/mass partials and a one-workgroup kernel reduces
  * them and plans the next step; 0 = the last workgroup of the step kernel does
