@@ -117,23 +117,7 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
 __global__ __launch_bounds__(256) void k_ensemble_finish(const Member* __restrict__ mem,
                                                         EnsPartials P) {
   const Member& mb = mem[blockIdx.x];
-  LoopControl* ctl = mb.ctl;
-  if (ctl->done) return;
-  float inv = 0.0f;
-  double vol = 0.0, clip = 0.0, bnd = 0.0;
-  int nf = 0;
-  for (int k = threadIdx.x; k < mb.n_waves; k += 256) {
-    const int q = mb.wave_base + k;
-    inv = fmaxf(inv, P.inv[q]);
-    vol += P.vol[q];
-    clip += P.clip[q];
-    bnd += P.bnd[q];
-    nf |= P.nf[q];
-  }
-  const Totals tot = block_reduce(inv, vol, clip, bnd, nf);
-  const double t_new = ctl->t + ctl->dt;
-  const int rj = rain_bracket(mb.L, t_new);
-  if (threadIdx.x == 0) finish_step(ctl, mb.L, tot, t_new, rj);
+  finish_block<256, 4>(mb.ctl, mb.L, P, mb.wave_base, mb.n_waves);
 }
 
 }  // namespace swe

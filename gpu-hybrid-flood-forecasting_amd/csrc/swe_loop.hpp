@@ -206,4 +206,73 @@ __device__ inline void finish_step(LoopControl* ctl, const LoopConst& L, const T
   __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Series a finish kernel stages in LDS when they are this short or shorter.
+constexpr int kLdsSeries = 512;
+
+// Finish of one step by one workgroup of BLOCK threads: reduce the partials
+// [first, first + count) of the step kernel that just ended (visible at the
+// kernel boundary), close the step in FP64 and plan the next one.  Every
+// global load is issued up front, in one round trip: the control block and
+// the rain and output-time series go to LDS while the partials are in
+// flight, so the single-thread bookkeeping afterwards reads only LDS.
+template <int BLOCK, int UNROLL, class Parts>
+__device__ void finish_block(LoopControl* __restrict__ ctl, const LoopConst& L, const Parts& P,
+                             int first, int count) {
+  __shared__ LoopControl s_ctl;
+  __shared__ double s_rt[kLdsSeries], s_rr[kLdsSeries], s_tm[kLdsSeries];
+  static_assert(sizeof(LoopControl) % sizeof(unsigned) == 0, "LoopControl is copied by words");
+  constexpr int kWords = (int)(sizeof(LoopControl) / sizeof(unsigned));
+  static_assert(kWords <= BLOCK, "one word per thread");
+  const int tid = threadIdx.x;
+  if (tid < kWords) reinterpret_cast<unsigned*>(&s_ctl)[tid] = reinterpret_cast<const unsigned*>(ctl)[tid];
+  const bool rain_lds = L.n_rain <= kLdsSeries, times_lds = L.n_times <= kLdsSeries;
+  if (rain_lds)
+    for (int k = tid; k < L.n_rain; k += BLOCK) {
+      s_rt[k] = L.rain_t[k];
+      s_rr[k] = L.rain_r[k];
+    }
+  if (times_lds)
+    for (int k = tid; k < L.n_times; k += BLOCK) s_tm[k] = L.times[k];
+  float inv = 0.0f;
+  double vol = 0.0, clip = 0.0, bnd = 0.0;
+  int nf = 0;
+  for (int base = tid; base < count; base += BLOCK * UNROLL) {
+    float pi[UNROLL];
+    double pv[UNROLL], pc[UNROLL], pb[UNROLL];
+    int pn[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      const int k = base + u * BLOCK;
+      const bool in = k < count;
+      const int q = first + (in ? k : 0);
+      pi[u] = in ? P.inv[q] : 0.0f;
+      pv[u] = in ? P.vol[q] : 0.0;
+      pc[u] = in ? P.clip[q] : 0.0;
+      pb[u] = in ? P.bnd[q] : 0.0;
+      pn[u] = in ? P.nf[q] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      inv = fmaxf(inv, pi[u]);
+      vol += pv[u];
+      clip += pc[u];
+      bnd += pb[u];
+      nf |= pn[u];
+    }
+  }
+  const Totals tot = block_reduce<BLOCK / kWave>(inv, vol, clip, bnd, nf);  // syncs the LDS copies
+  if (s_ctl.done) return;  // block-uniform
+  LoopConst Ls = L;
+  if (rain_lds) {
+    Ls.rain_t = s_rt;
+    Ls.rain_r = s_rr;
+  }
+  if (times_lds) Ls.times = s_tm;
+  const double t_new = s_ctl.t + s_ctl.dt;
+  const int rj = rain_bracket(Ls, t_new);
+  if (tid == 0) finish_step(&s_ctl, Ls, tot, t_new, rj);
+  __syncthreads();
+  if (tid < kWords) reinterpret_cast<unsigned*>(ctl)[tid] = reinterpret_cast<const unsigned*>(&s_ctl)[tid];
+}
+
 }  // namespace swe

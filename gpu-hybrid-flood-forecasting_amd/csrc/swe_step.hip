@@ -93,45 +93,15 @@ __global__ __launch_bounds__(kBlock) void k_step(
 // CFL maximum / volume of a state; MODE RED_INIT primes the loop.
 // ---------------------------------------------------------------------------
 // Finish of MODE_LOOP_SPLIT: one workgroup reduces the step kernel's
-// partials (visible at the kernel boundary) and plans the next step.  It is
-// on the critical path of every step, so it is wide and issues all its loads
-// before it adds: kFinishUnroll partials per thread per round trip.
-constexpr int kFinishBlock = 1024;
-constexpr int kFinishUnroll = 8;
+// partials and plans the next step.  It is on the critical path of every
+// step, so it is wide, loads kFinishUnroll partials per thread per round
+// trip, and issues all of its global loads at once (finish_block).
+constexpr int kFinishBlock = 256;
+constexpr int kFinishUnroll = 16;
 
 __global__ __launch_bounds__(kFinishBlock) void k_finish(LoopControl* __restrict__ ctl,
                                                          LoopConst L, Partials P, int nparts) {
-  if (ctl->done) return;
-  float inv = 0.0f;
-  double vol = 0.0, clip = 0.0, bnd = 0.0;
-  int nf = 0;
-  for (int base = threadIdx.x; base < nparts; base += kFinishBlock * kFinishUnroll) {
-    float pi[kFinishUnroll];
-    double pv[kFinishUnroll], pc[kFinishUnroll], pb[kFinishUnroll];
-    int pn[kFinishUnroll];
-#pragma unroll
-    for (int u = 0; u < kFinishUnroll; ++u) {
-      const int k = base + u * kFinishBlock;
-      const bool in = k < nparts;
-      pi[u] = in ? P.inv[k] : 0.0f;
-      pv[u] = in ? P.vol[k] : 0.0;
-      pc[u] = in ? P.clip[k] : 0.0;
-      pb[u] = in ? P.bnd[k] : 0.0;
-      pn[u] = in ? P.nf[k] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < kFinishUnroll; ++u) {
-      inv = fmaxf(inv, pi[u]);
-      vol += pv[u];
-      clip += pc[u];
-      bnd += pb[u];
-      nf |= pn[u];
-    }
-  }
-  const Totals tot = block_reduce<kFinishBlock / kWave>(inv, vol, clip, bnd, nf);
-  const double t_new = ctl->t + ctl->dt;
-  const int rj = rain_bracket(L, t_new);
-  if (threadIdx.x == 0) finish_step(ctl, L, tot, t_new, rj);
+  finish_block<kFinishBlock, kFinishUnroll>(ctl, L, P, 0, nparts);
 }
 
 template <class M, int MODE>

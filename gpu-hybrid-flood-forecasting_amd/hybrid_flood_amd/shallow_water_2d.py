@@ -106,3 +106,18 @@ class ShallowWaterSolver:
                 f"t={float(result.final_time_s):.3f}s."
             )
         return result
+
+
+def load_structured_grid(dem_path, roughness_path, domain_path, *, resolution_m: float = 50.0):
+    """GeoTIFF DEM + GeoPackage roughness and domain onto one grid
+    (shallow_water_2d.py:72-181); implemented in ``ingestion``."""
+    from .ingestion import load_structured_grid as _load
+
+    return _load(dem_path, roughness_path, domain_path, resolution_m=resolution_m)
+
+
+def load_rainfall_series(rainfall_path, *, scenario: str, default_rate_mm_hr: float = 0.0):
+    """Parquet mm/h rain series in m/s (shallow_water_2d.py:184-207); see ``ingestion``."""
+    from .ingestion import load_rainfall_series as _load
+
+    return _load(rainfall_path, scenario=scenario, default_rate_mm_hr=default_rate_mm_hr)

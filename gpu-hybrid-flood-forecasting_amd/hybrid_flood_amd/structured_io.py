@@ -24,6 +24,7 @@ import torch
 
 from . import _lib
 from .numerics import ScanResult, SWEParams, context_for
+from .shallow_water_2d import load_rainfall_series, load_structured_grid  # noqa: F401  (structured_io.py:44-157)
 
 VARIABLE_UNITS = {"elevation": "m", "depth": "m", "x_velocity": "m s-1", "y_velocity": "m s-1",
                   "velocity": "m s-1"}

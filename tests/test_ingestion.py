@@ -1,0 +1,427 @@
+"""Structured-grid ingestion (jax_solver/shallow_water_2d.py:49-207) without GDAL.
+
+The file readers are checked against independent writers (PIL for plain
+TIFF strips, hand-assembled bytes for the floating-point predictor and WKB)
+and the raster operations against closed forms and brute force.
+"""
+
+from __future__ import annotations
+
+import sqlite3
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+from hybrid_flood_amd import ingestion as ig
+
+
+# ---------------------------------------------------------------------------
+# Test writers (TIFF 6.0 + GeoTIFF tags; GeoPackage)
+# ---------------------------------------------------------------------------
+def _encode_pred3(block: np.ndarray) -> bytes:
+    """Floating-point predictor (TIFF Technote 3) of a (rows, cols) float32 block."""
+    rows, cols = block.shape
+    be = block.astype(">f4").view(np.uint8).reshape(rows, cols, 4)
+    planes = be.transpose(0, 2, 1).reshape(rows, 4 * cols).astype(np.int16)
+    diff = planes.copy()
+    diff[:, 1:] = planes[:, 1:] - planes[:, :-1]
+    return (diff & 0xFF).astype(np.uint8).tobytes()
+
+
+def write_geotiff(path, data, *, west, north, res, epsg=32643, nodata=None, tile=16,
+                  predictor=3):
+    """Little-endian tiled float32 GeoTIFF, deflate, predictor 1 or 3."""
+    data = np.asarray(data, np.float32)
+    h, w = data.shape
+    across, down = (w + tile - 1) // tile, (h + tile - 1) // tile
+    blobs = []
+    for by in range(down):
+        for bx in range(across):
+            blk = np.zeros((tile, tile), np.float32)
+            part = data[by * tile:(by + 1) * tile, bx * tile:(bx + 1) * tile]
+            blk[:part.shape[0], :part.shape[1]] = part
+            raw = _encode_pred3(blk) if predictor == 3 else blk.astype("<f4").tobytes()
+            blobs.append(zlib.compress(raw))
+    geokeys = [1, 1, 0, 3, 1024, 0, 1, 1, 1025, 0, 1, 1, 3072, 0, 1, epsg]
+    entries = [
+        (256, 3, [w]), (257, 3, [h]), (258, 3, [32]), (259, 3, [8]), (262, 3, [1]),
+        (277, 3, [1]), (284, 3, [1]), (317, 3, [predictor]), (322, 3, [tile]), (323, 3, [tile]),
+        (324, 4, None), (325, 4, [len(b) for b in blobs]), (339, 3, [3]),
+        (33550, 12, [res, res, 0.0]), (33922, 12, [0.0, 0.0, 0.0, west, north, 0.0]),
+        (34735, 3, geokeys),
+    ]
+    if nodata is not None:
+        entries.append((42113, 2, (repr(float(nodata)) + "\0").encode()))
+    entries.sort(key=lambda e: e[0])
+    fmt = {3: "H", 4: "I", 12: "d", 2: None}
+    size = {3: 2, 4: 4, 12: 8, 2: 1}
+    n = len(entries)
+    ifd_off = 8
+    data_off = ifd_off + 2 + 12 * n + 4
+    extra = bytearray()
+    tiles_off = None
+    ifd = bytearray(struct.pack("<H", n))
+    # Tile data goes after the out-of-line values; offsets are patched below.
+    pending = []
+    for tag, typ, vals in entries:
+        if tag == 324:
+            count = len(blobs)
+        elif typ == 2:
+            count = len(vals)
+        else:
+            count = len(vals)
+        nbytes = size[typ] * count
+        if tag == 324:
+            raw = b"\0" * nbytes
+        elif typ == 2:
+            raw = bytes(vals)
+        else:
+            raw = struct.pack("<" + fmt[typ] * count, *vals)
+        if nbytes <= 4:
+            ifd += struct.pack("<HHI", tag, typ, count) + raw.ljust(4, b"\0")
+            if tag == 324:
+                pending.append(("inline", len(ifd) - 4))
+        else:
+            off = data_off + len(extra)
+            ifd += struct.pack("<HHII", tag, typ, count, off)
+            if tag == 324:
+                tiles_off = off
+            extra += raw
+            if len(extra) % 2:
+                extra += b"\0"
+    ifd += struct.pack("<I", 0)
+    body = bytearray(b"II*\0" + struct.pack("<I", ifd_off)) + ifd + extra
+    offsets = []
+    for b in blobs:
+        offsets.append(len(body))
+        body += b
+    if tiles_off is not None:
+        body[tiles_off:tiles_off + 4 * len(blobs)] = struct.pack("<" + "I" * len(blobs), *offsets)
+    else:
+        pos = pending[0][1]
+        body[8 + pos:8 + pos + 4] = struct.pack("<I", offsets[0])
+    with open(path, "wb") as f:
+        f.write(bytes(body))
+
+
+def _wkb_polygon(rings) -> bytes:
+    out = struct.pack("<BII", 1, 3, len(rings))
+    for r in rings:
+        r = np.asarray(r, np.float64)
+        if not np.array_equal(r[0], r[-1]):
+            r = np.vstack([r, r[:1]])
+        out += struct.pack("<I", len(r)) + r.astype("<f8").tobytes()
+    return out
+
+
+def _wkb_multipolygon(polys) -> bytes:
+    return struct.pack("<BII", 1, 6, len(polys)) + b"".join(_wkb_polygon(p) for p in polys)
+
+
+def _wkb_linestring(pts) -> bytes:
+    pts = np.asarray(pts, np.float64)
+    return struct.pack("<BII", 1, 2, len(pts)) + pts.astype("<f8").tobytes()
+
+
+def _gp(wkb: bytes, srs=1) -> bytes:
+    return b"GP" + bytes([0, 0b00000001]) + struct.pack("<i", srs) + wkb  # no envelope, LE
+
+
+def write_gpkg(path, geoms, attrs=None, *, epsg=32643, projected=True):
+    con = sqlite3.connect(path)
+    con.execute("CREATE TABLE gpkg_spatial_ref_sys (srs_name TEXT, srs_id INTEGER PRIMARY KEY, "
+                "organization TEXT, organization_coordsys_id INTEGER, definition TEXT, "
+                "description TEXT)")
+    wkt = ('PROJCS["WGS 84 / UTM zone 43N"]' if projected else 'GEOGCS["WGS 84"]')
+    con.execute("INSERT INTO gpkg_spatial_ref_sys VALUES (?,?,?,?,?,?)",
+                ("crs", 1, "EPSG", epsg, wkt, ""))
+    con.execute("CREATE TABLE gpkg_contents (table_name TEXT, data_type TEXT, srs_id INTEGER)")
+    con.execute("CREATE TABLE gpkg_geometry_columns (table_name TEXT, column_name TEXT, "
+                "geometry_type_name TEXT, srs_id INTEGER, z INTEGER, m INTEGER)")
+    con.execute("INSERT INTO gpkg_contents VALUES ('layer', 'features', 1)")
+    con.execute("INSERT INTO gpkg_geometry_columns VALUES ('layer', 'geom', 'GEOMETRY', 1, 0, 0)")
+    attrs = attrs or {}
+    cols = "".join(f', "{k}" REAL' for k in attrs)
+    con.execute(f'CREATE TABLE layer (fid INTEGER PRIMARY KEY, geom BLOB{cols})')
+    for i, g in enumerate(geoms):
+        vals = [attrs[k][i] for k in attrs]
+        q = ",".join("?" * (1 + len(vals)))
+        names = ",".join(["geom"] + [f'"{k}"' for k in attrs])
+        con.execute(f"INSERT INTO layer ({names}) VALUES ({q})", [_gp(g)] + vals)
+    con.commit()
+    con.close()
+
+
+# ---------------------------------------------------------------------------
+# TIFF reader
+# ---------------------------------------------------------------------------
+def test_predictor3_decodes_hand_assembled_row():
+    # Row [1.0, 2.0]: big-endian 3F800000 40000000 -> byte planes
+    # 3F 40 | 80 00 | 00 00 | 00 00 -> differenced 3F 01 40 80 00 00 00 00.
+    enc = np.array([[0x3F, 0x01, 0x40, 0x80, 0, 0, 0, 0]], np.uint8)
+    got = ig._undo_predictor(enc, 3, np.dtype("<f4"), 1)
+    np.testing.assert_array_equal(got, [[1.0, 2.0]])
+    assert _encode_pred3(np.array([[1.0, 2.0]], np.float32)) == enc.tobytes()
+
+
+@pytest.mark.parametrize("compression", [None, "tiff_deflate", "tiff_lzw"])
+def test_tiff_reader_matches_pil_strips(tmp_path, compression):
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(3)
+    arr = rng.normal(size=(37, 53)).astype(np.float32)
+    p = tmp_path / "a.tif"
+    try:
+        Image.fromarray(arr, mode="F").save(p, compression=compression)
+    except (OSError, ValueError, KeyError) as e:  # PIL built without this codec
+        pytest.skip(f"PIL cannot write {compression}: {e}")
+    r = ig.read_geotiff(p)
+    np.testing.assert_array_equal(r.data, arr)
+    assert r.nodata is None and r.crs is None
+
+
+def test_geotiff_tiled_deflate_predictor3(tmp_path):
+    rng = np.random.default_rng(4)
+    arr = (100 + 10 * rng.normal(size=(37, 45))).astype(np.float32)
+    arr[3, 7] = -9999.0
+    p = tmp_path / "dem.tif"
+    write_geotiff(p, arr, west=500.0, north=2000.0, res=2.5, nodata=-9999.0, tile=16)
+    r = ig.read_geotiff(p)
+    np.testing.assert_array_equal(r.data, arr)
+    assert tuple(r.transform) == (2.5, 0.0, 500.0, 0.0, -2.5, 2000.0)
+    assert r.nodata == -9999.0 and r.crs == "EPSG:32643" and r.projected
+
+
+# ---------------------------------------------------------------------------
+# GeoPackage reader
+# ---------------------------------------------------------------------------
+def test_wkb_polygon_hand_assembled():
+    wkb = bytes.fromhex("0103000000010000000400000000000000000000000000000000000000000000000000"
+                        "F03F0000000000000000000000000000F03F000000000000F03F"
+                        "00000000000000000000000000000000")
+    kind, rings, end = ig._wkb_geometry(wkb)
+    assert kind == "Polygon" and end == len(wkb)
+    np.testing.assert_array_equal(rings[0], [[0, 0], [1, 0], [1, 1], [0, 0]])
+
+
+def test_geopackage_features_and_crs(tmp_path):
+    sq = [[(0, 0), (10, 0), (10, 10), (0, 10)], [(2, 2), (4, 2), (4, 4), (2, 4)]]
+    mp = [[[(20, 0), (25, 0), (25, 5)]], [[(30, 0), (35, 0), (35, 5)]]]
+    line = [(0, 20), (5, 20), (5, 25), (0, 20)]
+    p = tmp_path / "l.gpkg"
+    write_gpkg(p, [_wkb_polygon(sq), _wkb_multipolygon(mp), _wkb_linestring(line)],
+               {"roughness": [0.03, 0.05, 0.1]})
+    layer = ig.read_geopackage(p)
+    assert layer.crs == "EPSG:32643" and layer.projected
+    assert layer.kinds == ["Polygon", "MultiPolygon", "LineString"]
+    assert layer.attributes["roughness"] == [0.03, 0.05, 0.1]
+    assert len(layer.polygons[0]) == 2 and len(layer.polygons[1]) == 2 and len(layer.polygons[2]) == 1
+    np.testing.assert_array_equal(layer.polygons[0][1][:4], sq[1])
+
+
+# ---------------------------------------------------------------------------
+# Geometry and burning
+# ---------------------------------------------------------------------------
+def test_aligned_grid_geometry_closed_form():
+    x, y, tr, shape = ig.aligned_grid_geometry((103.0, 51.0, 390.0, 260.0), 50.0)
+    assert shape == (5, 6)  # x: 100..400, y: 50..300
+    assert tuple(tr) == (50.0, 0.0, 100.0, 0.0, -50.0, 300.0)
+    np.testing.assert_array_equal(x, 125.0 + 50.0 * np.arange(6))
+    np.testing.assert_array_equal(y, 75.0 + 50.0 * np.arange(5))
+
+
+def _centres(shape, tr):
+    rows, cols = np.mgrid[0:shape[0], 0:shape[1]]
+    return tr.c + (cols + 0.5) * tr.a, tr.f + (rows + 0.5) * tr.e
+
+
+def test_polygon_cells_rectangle_and_hole():
+    tr = ig.NorthUpTransform.from_origin(0.0, 40.0, 1.0, 1.0)
+    shape = (40, 30)
+    outer = np.array([(3.2, 5.7), (21.9, 5.7), (21.9, 33.1), (3.2, 33.1)])
+    hole = np.array([(8.1, 10.4), (12.6, 10.4), (12.6, 20.2), (8.1, 20.2)])
+    m = ig.polygon_cells([outer, hole[::-1]], shape, tr)
+    X, Y = _centres(shape, tr)
+    want = (X > 3.2) & (X < 21.9) & (Y > 5.7) & (Y < 33.1)
+    want &= ~((X > 8.1) & (X < 12.6) & (Y > 10.4) & (Y < 20.2))
+    np.testing.assert_array_equal(m, want)
+
+
+def test_polygon_cells_triangle_brute_force():
+    tr = ig.NorthUpTransform.from_origin(100.0, 300.0, 2.0, 2.0)
+    shape = (90, 70)
+    tri = np.array([(103.3, 121.7), (231.1, 140.9), (150.7, 291.3)])
+    m = ig.polygon_cells([tri], shape, tr)
+    X, Y = _centres(shape, tr)
+
+    def side(a, b):
+        return (b[0] - a[0]) * (Y - a[1]) - (b[1] - a[1]) * (X - a[0])
+
+    s0, s1, s2 = side(tri[0], tri[1]), side(tri[1], tri[2]), side(tri[2], tri[0])
+    want = ((s0 > 0) & (s1 > 0) & (s2 > 0)) | ((s0 < 0) & (s1 < 0) & (s2 < 0))
+    np.testing.assert_array_equal(m, want)
+    assert m.sum() > 500
+
+
+def test_rasterize_later_shapes_win_and_fill():
+    tr = ig.NorthUpTransform.from_origin(0.0, 10.0, 1.0, 1.0)
+    a = [np.array([(0.1, 0.1), (6.2, 0.1), (6.2, 9.9), (0.1, 9.9)])]
+    b = [np.array([(4.1, 0.1), (9.9, 0.1), (9.9, 9.9), (4.1, 9.9)])]
+    out = ig.rasterize([(a, 1.0), (b, 2.0)], (10, 10), tr, fill=np.nan)
+    assert np.all(out[:, :4] == 1.0) and np.all(out[:, 4:] == 2.0)
+    out = ig.rasterize([(a, 1.0)], (10, 10), tr, fill=np.nan)
+    assert np.all(np.isnan(out[:, 6:])) and np.all(out[:, :6] == 1.0)
+
+
+# ---------------------------------------------------------------------------
+# Resampling and hole filling
+# ---------------------------------------------------------------------------
+def _plane(X, Y):
+    return 0.25 * X - 0.4 * Y + 7.0
+
+
+@pytest.mark.parametrize("dst_res", [5.0, 4.0, 0.5])
+def test_bilinear_reproduces_a_plane(dst_res):
+    src_tr = ig.NorthUpTransform.from_origin(0.0, 200.0, 1.0, 1.0)
+    X, Y = _centres((200, 200), src_tr)
+    src = _plane(X, Y)
+    dst_tr = ig.NorthUpTransform.from_origin(40.0, 160.0, dst_res, dst_res)
+    shape = (int(80 / dst_res), int(80 / dst_res))
+    out = ig.reproject_same_crs(src, src_tr, None, shape, dst_tr, resampling="bilinear")
+    DX, DY = _centres(shape, dst_tr)
+    np.testing.assert_allclose(out, _plane(DX, DY), rtol=0, atol=2e-4)
+
+
+def test_bilinear_renormalises_over_nodata_and_marks_outside():
+    src_tr = ig.NorthUpTransform.from_origin(0.0, 50.0, 1.0, 1.0)
+    src = np.full((50, 50), 3.5)
+    src[10:14, 20:30] = -9999.0
+    src[30, 5] = np.nan
+    dst_tr = ig.NorthUpTransform.from_origin(-10.0, 50.0, 5.0, 5.0)
+    out = ig.reproject_same_crs(src, src_tr, -9999.0, (10, 14), dst_tr)
+    assert np.all(np.isnan(out[:, :2])) and np.all(np.isnan(out[:, 12:]))
+    np.testing.assert_allclose(out[:, 2:12], 3.5, rtol=0, atol=1e-6)
+
+
+def test_nearest_takes_the_containing_cell():
+    src_tr = ig.NorthUpTransform.from_origin(0.0, 30.0, 3.0, 3.0)
+    src = np.arange(100, dtype=np.float64).reshape(10, 10)
+    dst_tr = ig.NorthUpTransform.from_origin(1.0, 29.0, 2.0, 2.0)
+    out = ig.reproject_same_crs(src, src_tr, None, (14, 14), dst_tr, resampling="nearest")
+    DX, DY = _centres((14, 14), dst_tr)
+    ci = np.floor(DX / 3.0).astype(int)
+    ri = np.floor((30.0 - DY) / 3.0).astype(int)
+    np.testing.assert_array_equal(out, src[ri, ci].astype(np.float32))
+
+
+def test_fill_from_nearest_brute_force():
+    rng = np.random.default_rng(8)
+    vals = rng.normal(size=(23, 19)).astype(np.float32)
+    holes = rng.random(vals.shape) < 0.3
+    vals[holes] = np.nan
+    out = ig.fill_from_nearest(vals, holes)
+    fr, fc = np.nonzero(~holes)
+    for r, c in zip(*np.nonzero(holes)):
+        d = (fr - r) ** 2 + (fc - c) ** 2
+        best = d.min()
+        cands = vals[fr[d == best], fc[d == best]]
+        assert out[r, c] in cands
+    with pytest.raises(ValueError, match="no finite elevations"):
+        ig.fill_from_nearest(np.full((3, 3), np.nan, np.float32), np.ones((3, 3), bool))
+
+
+# ---------------------------------------------------------------------------
+# Reference entry points
+# ---------------------------------------------------------------------------
+def test_grid_from_rasters_flips_and_zeroes():
+    mask = np.array([[0, 1, 1], [1, 1, 0]], bool)
+    bed = np.array([[9.0, np.nan, 2.0], [3.0, 4.0, 9.0]], np.float32)
+    near = np.array([[0.0, 5.0, 0.0], [0.0, 0.0, 0.0]], np.float32)
+    rough = np.array([[np.nan, 0.1, 0.2], [0.3, 0.4, np.nan]], np.float32)
+    g = ig.grid_from_rasters(mask, bed, near, rough, [0.5, 1.5, 2.5], [0.5, 1.5], 1.0)
+    np.testing.assert_array_equal(g.domain_mask, mask[::-1])
+    np.testing.assert_array_equal(g.bed, [[3.0, 4.0, 0.0], [0.0, 5.0, 2.0]])
+    np.testing.assert_array_equal(g.manning_n, np.float32([[0.3, 0.4, 0.0], [0.0, 0.1, 0.2]]))
+    rough[0, 1] = np.nan
+    with pytest.raises(ValueError, match="left 1 domain cells unassigned"):
+        ig.grid_from_rasters(mask, bed, near, rough, [0, 1, 2], [0, 1], 1.0)
+
+
+def test_load_structured_grid_end_to_end(tmp_path):
+    # DEM: plane on a 2 m grid with a no-data hole; domain: a 200 x 120 m box;
+    # roughness: two strips.  Grid resolution 10 m.
+    src_tr = ig.NorthUpTransform.from_origin(1000.0, 5200.0, 2.0, 2.0)
+    X, Y = _centres((150, 140), src_tr)
+    dem = _plane(X, Y).astype(np.float32)
+    dem[60:64, 60:64] = -9999.0
+    write_geotiff(tmp_path / "dem.tif", dem, west=1000.0, north=5200.0, res=2.0, nodata=-9999.0)
+    box = [(1020.0, 4960.0), (1220.0, 4960.0), (1220.0, 5080.0), (1020.0, 5080.0)]
+    write_gpkg(tmp_path / "domain.gpkg", [_wkb_polygon([box])])
+    west = [(1000.0, 4900.0), (1121.0, 4900.0), (1121.0, 5200.0), (1000.0, 5200.0)]
+    east = [(1119.0, 4900.0), (1300.0, 4900.0), (1300.0, 5200.0), (1119.0, 5200.0)]
+    write_gpkg(tmp_path / "rough.gpkg", [_wkb_polygon([west]), _wkb_polygon([east])],
+               {"roughness": [0.02, 0.06]})
+    g = ig.load_structured_grid(tmp_path / "dem.tif", tmp_path / "rough.gpkg",
+                                tmp_path / "domain.gpkg", resolution_m=10.0)
+    assert g.bed.shape == (12, 20) and g.domain_mask.all()
+    np.testing.assert_array_equal(g.x, 1025.0 + 10.0 * np.arange(20))
+    np.testing.assert_array_equal(g.y, 4965.0 + 10.0 * np.arange(12))
+    # South-to-north rows: bed[j, i] is the plane at (x[i], y[j]) wherever the
+    # tent (radius 10 m) misses the no-data hole (x 1120-1128, y 5072-5080);
+    # next to it the weights are renormalised over the valid cells only.
+    want = _plane(g.x[None, :], g.y[:, None])
+    far = ((np.abs(g.x[None, :] - 1124.0) > 14.0) | (np.abs(g.y[:, None] - 5076.0) > 14.0))
+    np.testing.assert_allclose(g.bed[far], want[far], atol=2e-4)
+    assert np.all(np.isfinite(g.bed)) and np.abs(g.bed - want).max() < 1.0
+    np.testing.assert_array_equal(g.manning_n[:, g.x < 1120.0], np.float32(0.02))
+    np.testing.assert_array_equal(g.manning_n[:, g.x > 1120.0], np.float32(0.06))
+    assert g.crs == "EPSG:32643"
+    with pytest.raises(ValueError, match="resolution must be positive"):
+        ig.load_structured_grid(tmp_path / "dem.tif", tmp_path / "rough.gpkg",
+                                tmp_path / "domain.gpkg", resolution_m=0.0)
+    write_gpkg(tmp_path / "geo.gpkg", [_wkb_polygon([box])], projected=False, epsg=4326)
+    with pytest.raises(ValueError, match="projected CRS"):
+        ig.load_structured_grid(tmp_path / "dem.tif", tmp_path / "rough.gpkg",
+                                tmp_path / "geo.gpkg", resolution_m=10.0)
+    write_gpkg(tmp_path / "norough.gpkg", [_wkb_polygon([west])], {"n": [0.1]})
+    with pytest.raises(ValueError, match="'roughness' attribute"):
+        ig.load_structured_grid(tmp_path / "dem.tif", tmp_path / "norough.gpkg",
+                                tmp_path / "domain.gpkg", resolution_m=10.0)
+
+
+def test_load_rainfall_series(tmp_path):
+    pd = pytest.importorskip("pandas")
+    pytest.importorskip("pyarrow")
+    stamps = pd.date_range("2024-07-01 00:00", periods=5, freq="15min")
+    df = pd.DataFrame({
+        "timestamp": list(stamps[::-1]) + list(stamps),
+        "scenario": ["wet"] * 5 + ["dry"] * 5,
+        "rainfall_mm_hr": [40.0, 30.0, 20.0, 10.0, 0.0] + [0.0] * 5,
+        "is_missing_timestamp": [False] * 10,
+        "units": ["mm/hr"] * 10,
+    })
+    p = tmp_path / "rain.parquet"
+    df.to_parquet(p)
+    t, r, d = ig.load_rainfall_series(p, scenario="wet", default_rate_mm_hr=3.6)
+    np.testing.assert_array_equal(t, np.float32([0, 900, 1800, 2700, 3600]))
+    assert t.dtype == np.float32 and r.dtype == np.float32
+    np.testing.assert_array_equal(r, np.float32([0, 10, 20, 30, 40]) * np.float32(1e-3 / 3600))
+    assert d == pytest.approx(1e-6)
+    with pytest.raises(ValueError, match="unavailable"):
+        ig.load_rainfall_series(p, scenario="none")
+    df2 = df.copy()
+    df2.loc[0, "units"] = "in/hr"
+    df2.to_parquet(p)
+    with pytest.raises(ValueError, match="Expected rainfall units"):
+        ig.load_rainfall_series(p, scenario="wet")
+    df.drop(columns=["units"]).to_parquet(p)
+    with pytest.raises(ValueError, match="missing columns"):
+        ig.load_rainfall_series(p, scenario="wet")
+
+
+def test_reference_module_paths_reexport_the_loaders():
+    from hybrid_flood_amd import shallow_water_2d, structured_io
+
+    assert callable(shallow_water_2d.load_structured_grid)
+    assert structured_io.load_rainfall_series is shallow_water_2d.load_rainfall_series
