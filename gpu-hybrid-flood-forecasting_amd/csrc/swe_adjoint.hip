@@ -42,6 +42,9 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
     const T* __restrict__ LHV, T* __restrict__ OH, T* __restrict__ OHU, T* __restrict__ OHV,
     T* __restrict__ grad_n, T* __restrict__ grad_mult, float dt, float dtrate, float dtg,
     int rows, int segs_x, int n_waves) {
+  // Hardware reciprocal/sqrt/log2/exp2 in the reverse pieces of the fast
+  // variant's float sweep (adj::quot and friends).
+  constexpr bool kFastAdj = M::kFast && sizeof(T) == sizeof(float);
   const int gw = blockIdx.x * (blockDim.x / kWave) + wave_in_block();
   if (gw >= n_waves) return;
   const int lane = threadIdx.x & (kWave - 1);
@@ -105,7 +108,7 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
       hs = hs + dtrate * mult;
       // ---- reverse of the pointwise part ----
       const adj::PointVjp<T> pv =
-          adj::point_vjp<T>(dry, T(kPow73), T(hs), T(hus), T(hvs), T(G.n[idx]), Rcur.a,
+          adj::point_vjp<T, kFastAdj>(dry, T(kPow73), T(hs), T(hus), T(hvs), T(G.n[idx]), Rcur.a,
                             T(dtrate), T(dtg), LH[idx], LHU[idx], LHV[idx]);
       if (real) {
         ls0 = pv.lh;
@@ -122,7 +125,7 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
       // ---- reverse of this lane's west x face (A = west cell, B = this cell) ----
       const T lw0 = from_west(ls0), lw1 = from_west(ls1), lw2 = from_west(ls2);
       adj::SideGrad<T> dA, dB;
-      adj::face_vjp<true, T>(g, hg, dry, T(wh), T(whu), T(whv), T(wb), wa, T(Rcur.h),
+      adj::face_vjp<true, T, kFastAdj>(g, hg, dry, T(wh), T(whu), T(whv), T(wb), wa, T(Rcur.h),
                              T(Rcur.hu), T(Rcur.hv), T(Rcur.b), Rcur.a, cx * (ls0 - lw0),
                              cx * (ls1 - lw1), cx * (ls2 - lw2), -cx * lw1, cx * ls1, dA, dB);
       // x: (h, q_n, q_t) = (h, hu, hv)
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
     if (r >= j0) {
       adj::SideGrad<T> dA, dB;
       // y: normal momentum hv, tangential hu
-      adj::face_vjp<false, T>(g, hg, dry, T(Rprev.h), T(Rprev.hv), T(Rprev.hu), T(Rprev.b),
+      adj::face_vjp<false, T, kFastAdj>(g, hg, dry, T(Rprev.h), T(Rprev.hv), T(Rprev.hu), T(Rprev.b),
                               Rprev.a, T(Rcur.h), T(Rcur.hv), T(Rcur.hu), T(Rcur.b), Rcur.a,
                               cy * (ls0 - lsp0), cy * (ls2 - lsp2), cy * (ls1 - lsp1),
                               -cy * lsp2, cy * ls2, dA, dB);

@@ -48,6 +48,36 @@ struct SideGrad {
 template <class T>
 SWE_HD inline T tmax(T a, T b) { return a > b ? a : b; }
 
+// Arithmetic of the reverse pieces.  FAST (float on the GPU, for the fast
+// kernel variant) uses the hardware reciprocal, square root and log2/exp2,
+// ~1 ulp each, as the fast forward step does; otherwise (the host build that
+// finite differences check, double cotangents, the reference variant) the
+// operations are correctly rounded.  Quotients whose divisor can be
+// denormal stay exact in both.
+template <bool FAST, class T>
+SWE_HD inline T quot(T a, T b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (FAST && sizeof(T) == 4) return a * __builtin_amdgcn_rcpf(b);
+#endif
+  return a / b;
+}
+
+template <bool FAST, class T>
+SWE_HD inline T tsqrt(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (FAST && sizeof(T) == 4) return __builtin_amdgcn_sqrtf(x);
+#endif
+  return std::sqrt(x);
+}
+
+template <bool FAST, class T>
+SWE_HD inline T tpow(T x, T p) {  // x > 0
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (FAST && sizeof(T) == 4) return __builtin_amdgcn_exp2f(p * __builtin_amdgcn_logf(x));
+#endif
+  return std::pow(x, p);
+}
+
 // Face in face-normal coordinates: side A is west/south, q_n the normal and
 // q_t the tangential momentum of each cell (before mirroring).  XDIR: the x
 // face's tangential flux is q_n v_t, the y face's q_t v_n.
@@ -89,7 +119,7 @@ SWE_HD inline Face5<T> face_fwd(T g, T hg, T dry, T hA, T qnA, T qtA, T bA, int 
   return o;
 }
 
-template <bool XDIR, class T>
+template <bool XDIR, class T, bool FAST = false>
 SWE_HD inline void face_vjp(T g, T hg, T dry, T hA, T qnA, T qtA, T bA, int aA, T hB, T qnB,
                             T qtB, T bB, int aB, T gF0, T gFn, T gFt, T gcA, T gcB,
                             SideGrad<T>& dA, SideGrad<T>& dB) {
@@ -101,8 +131,8 @@ SWE_HD inline void face_vjp(T g, T hg, T dry, T hA, T qnA, T qtA, T bA, int aA, 
   if (wallA) { hB = hA; qnB = -qnA; qtB = qtA; bB = bA; }
   if (wallB) { hA = hB; qnA = -qnB; qtA = qtB; bA = bB; }
   const bool dwA = hA > dry, dwB = hB > dry;
-  const T uA = dwA ? qnA / hA : T(0), vA = dwA ? qtA / hA : T(0);
-  const T uB = dwB ? qnB / hB : T(0), vB = dwB ? qtB / hB : T(0);
+  const T uA = dwA ? quot<FAST>(qnA, hA) : T(0), vA = dwA ? quot<FAST>(qtA, hA) : T(0);
+  const T uB = dwB ? quot<FAST>(qnB, hB) : T(0), vB = dwB ? quot<FAST>(qtB, hB) : T(0);
   const T bf = tmax(bA, bB);
   const T eA = (hA + bA) - bf, eB = (hB + bB) - bf;
   const T hrA = eA > T(0) ? eA : T(0), hrB = eB > T(0) ? eB : T(0);
@@ -110,7 +140,7 @@ SWE_HD inline void face_vjp(T g, T hg, T dry, T hA, T qnA, T qtA, T bA, int aA, 
   const bool wA = hrA > dry, wB = hrB > dry;
   const T vnA = wA ? uA : T(0), vtA = wA ? vA : T(0);
   const T vnB = wB ? uB : T(0), vtB = wB ? vB : T(0);
-  const T cA = std::sqrt(g * hrA), cB = std::sqrt(g * hrB);
+  const T cA = tsqrt<FAST>(g * hrA), cB = tsqrt<FAST>(g * hrB);
   const T s1A = vnA - cA, s1B = vnB - cB;
   const bool minA = s1A <= s1B;
   const T sA = minA ? s1A : s1B;
@@ -132,7 +162,7 @@ SWE_HD inline void face_vjp(T g, T hg, T dry, T hA, T qnA, T qtA, T bA, int aA, 
   } else {
     const T spread = sB - sA;
     const bool clamped = !(spread > T(1.0e-12));
-    const T r = clamped ? T(1) : T(1) / spread;
+    const T r = clamped ? T(1) : quot<FAST>(T(1), spread);
     for (int k = 0; k < 3; ++k) {
       const T mid = ((sB * FA[k] - sA * FB[k]) + (sA * sB) * dU[k]) * r;
       gFA[k] = gF[k] * sB * r;
@@ -177,8 +207,8 @@ SWE_HD inline void face_vjp(T g, T hg, T dry, T hA, T qnA, T qtA, T bA, int aA, 
   // accumulated over a rollout, swamps the gradient that finite differences
   // of any practical size see.  Shallow cells (h <= dry) keep it: their thin
   // films change smoothly with the forcing.
-  if (cA > T(0) && (wA || !dwA)) ghrA += gcsA * g / (T(2) * cA);
-  if (cB > T(0) && (wB || !dwB)) ghrB += gcsB * g / (T(2) * cB);
+  if (cA > T(0) && (wA || !dwA)) ghrA += quot<FAST>(gcsA * g, T(2) * cA);
+  if (cB > T(0) && (wB || !dwB)) ghrB += quot<FAST>(gcsB * g, T(2) * cB);
   T guA = wA ? gvnA : T(0), gvA = wA ? gvtA : T(0);
   T guB = wB ? gvnB : T(0), gvB = wB ? gvtB : T(0);
   ghrA += gQnA * uA + gQtA * vA;
@@ -196,14 +226,14 @@ SWE_HD inline void face_vjp(T g, T hg, T dry, T hA, T qnA, T qtA, T bA, int aA, 
   if (eB > T(0)) ghB += ghrB;
   T gqnA = T(0), gqtA = T(0), gqnB = T(0), gqtB = T(0);
   if (dwA) {
-    gqnA = guA / hA;
-    gqtA = gvA / hA;
-    ghA -= (guA * uA + gvA * vA) / hA;
+    gqnA = quot<FAST>(guA, hA);
+    gqtA = quot<FAST>(gvA, hA);
+    ghA -= quot<FAST>(guA * uA + gvA * vA, hA);
   }
   if (dwB) {
-    gqnB = guB / hB;
-    gqtB = gvB / hB;
-    ghB -= (guB * uB + gvB * vB) / hB;
+    gqnB = quot<FAST>(guB, hB);
+    gqtB = quot<FAST>(gvB, hB);
+    ghB -= quot<FAST>(guB * uB + gvB * vB, hB);
   }
   // Undo the mirroring: the mirrored side is a function of the active side.
   if (wallA) {
@@ -242,29 +272,29 @@ SWE_HD inline PointOut<T> point_fwd(T dry, T p, T hs, T hus, T hvs, T n, int act
   return {hn, wet ? hus / D : T(0), wet ? hvs / D : T(0)};
 }
 
-template <class T>
+template <class T, bool FAST = false>
 SWE_HD inline PointVjp<T> point_vjp(T dry, T p, T hs, T hus, T hvs, T n, int active, T dtrate,
                                     T dtg, T lam_h, T lam_hu, T lam_hv) {
   PointVjp<T> r{T(0), T(0), T(0), T(0), T(0)};
   if (!active) return r;
   const T hn = hs > T(0) ? hs : T(0);
-  const T q = std::sqrt(hus * hus + hvs * hvs);
+  const T q = tsqrt<FAST>(hus * hus + hvs * hvs);
   const T hd = hn > dry ? hn : dry;
-  const T P = std::pow(hd, p);
-  const T K = (dtg * (n * n)) * q / P;
+  const T P = tpow<FAST>(hd, p);
+  const T K = quot<FAST>((dtg * (n * n)) * q, P);
   const T D = T(1) + K;
   const bool wet = hn > dry;
   const T lhu_n = wet ? lam_hu : T(0);
   const T lhv_n = wet ? lam_hv : T(0);
-  const T s = (lhu_n * hus + lhv_n * hvs) / (D * D);  // = -dL/dD
+  const T s = quot<FAST>(lhu_n * hus + lhv_n * hvs, D * D);  // = -dL/dD
   T lhn = lam_h;
-  T lhus = lhu_n / D, lhvs = lhv_n / D;
+  T lhus = quot<FAST>(lhu_n, D), lhvs = quot<FAST>(lhv_n, D);
   if (q > T(0)) {
-    const T lq = -s * K / q;  // dD/dq = K/q
+    const T lq = -s * K / q;  // dD/dq = K/q; q may be denormal: exact quotients
     lhus += lq * hus / q;
     lhvs += lq * hvs / q;
   }
-  if (hn > dry) lhn += s * p * K / hn;  // dD/dh = -p K / h
+  if (hn > dry) lhn += quot<FAST>(s * p * K, hn);  // dD/dh = -p K / h
   r.gn = (n != T(0)) ? -s * T(2) * K / n : T(0);
   const T lhs = hs > T(0) ? lhn : T(0);
   r.lh = lhs;
