@@ -226,21 +226,35 @@ def bench_adjoint(variant: str, n: int, steps: int, plain_cups: float):
     def rmse(d):
         return torch.sqrt(((d[1:].double() - target[1:].double()) ** 2).mean())
 
-    # plain forward of the same rollout
+    # Plain forward of the same rollout and forward + gradient, each after one
+    # untimed warm-up call (first-use allocations and graph capture excluded,
+    # as in the step benchmark), median of `reps` timed calls.
+    reps = 3
     pl = model.params_for(n0, 1.0)
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    res = nm.integrate_to_outputs(state0, pl, times, max_steps=4 * steps, variant=variant)
-    torch.cuda.synchronize(dev)
-    fwd_s = time.perf_counter() - t0
-    n_var = n0.clone().requires_grad_(True)
-    alpha = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    loss = rmse(rollout_depths(n_var, alpha, model))
-    loss.backward()
-    torch.cuda.synchronize(dev)
-    grad_s = time.perf_counter() - t0
+
+    def plain():
+        return nm.integrate_to_outputs(state0, pl, times, max_steps=4 * steps, variant=variant)
+
+    def gradient():
+        n_var = n0.clone().requires_grad_(True)
+        alpha = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+        loss = rmse(rollout_depths(n_var, alpha, model))
+        loss.backward()
+        return loss, n_var, alpha
+
+    def timed(fn):
+        out, ts = None, []
+        fn()
+        for _ in range(reps):
+            torch.cuda.synchronize(dev)
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize(dev)
+            ts.append(time.perf_counter() - t0)
+        return out, float(np.median(ts))
+
+    res, fwd_s = timed(plain)
+    (loss, n_var, alpha), grad_s = timed(gradient)
     K = res.steps_used
     cells = n * n
     # Frozen-schedule central differences (the map the adjoint differentiates).
@@ -259,7 +273,9 @@ def bench_adjoint(variant: str, n: int, steps: int, plain_cups: float):
     return {
         "workload": f"urban {n}x{n}, dx 2 m, dry start, 1-min rain series, {K} CFL steps "
                     f"({horizon:.0f} s), loss = depth RMSE at 10 output times vs a run with a "
-                    "hidden smooth n-field in [0.01, 0.06] and rain x1.3, sqrt(K) checkpoints",
+                    "hidden smooth n-field in [0.01, 0.06] and rain x1.3, checkpoints every "
+                    "ceil(sqrt(max_steps)) steps stored by the forward loop",
+        "timing": f"median of {reps} calls after one untimed warm-up call, each",
         "steps": K, "forward_s": fwd_s, "forward_plus_gradient_s": grad_s,
         "gradient_vs_forward_time": grad_s / fwd_s,
         "gradient_cell_steps_per_s": cells * K / grad_s,

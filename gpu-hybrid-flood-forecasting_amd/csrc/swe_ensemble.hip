@@ -330,6 +330,7 @@ __global__ __launch_bounds__(256) void k_ens_init(const Member* __restrict__ mem
     if (mb.L.n_times < 2 || mb.L.max_steps <= 0 || tot.nf || t0 >= mb.L.times[1] - kOutputSlack) {
       ctl->done = 1;
       ctl->rec_lo = ctl->rec_hi = 0;
+      ctl->ck_slot = -1;
     } else {
       ctl->done = 0;
       plan_next_step(ctl, mb.L, tot.inv, t0, 1, s_j);

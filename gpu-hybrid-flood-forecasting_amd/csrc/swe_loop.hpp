@@ -31,6 +31,7 @@ struct LoopControl {
   float dt_f, dtrate_f, dtg_f;  // float32(dt), float32(dt*rate), float32(dt*g)
   int done;
   int rec_lo, rec_hi;  // output slots the pending step writes
+  int ck_slot;         // checkpoint slot the pending step writes (-1: none)
   int steps, out_idx;
   double t, dt, rate;
   double vol0, vol, rain, bnd, clip;
@@ -45,7 +46,14 @@ struct LoopConst {
   const double* rain_r;
   const double* times;
   double* dt_log;  // optional: dt of step k is written to dt_log[k] (k < dt_log_cap)
+  // Optional checkpoints: the state after s * ck_every steps goes to slot s of
+  // [ck_h | ck_hu | ck_hv] (each ck_cap x N), unless that step also writes an
+  // output slot (the output cube then holds it).
+  float* ck_h;
+  float* ck_hu;
+  float* ck_hv;
   int n_rain, n_times, max_steps, dt_log_cap;
+  int ck_every, ck_cap;
   int record;      // 0: output times only stop/clamp the loop, no state is stored
 };
 
@@ -88,6 +96,12 @@ __device__ void plan_next_step(LoopControl* ctl, const LoopConst& L, float m, do
   ctl->dtg_f = (float)(dt * L.gravity);
   ctl->rec_lo = oi;
   ctl->rec_hi = L.record ? hi : oi;
+  const int after = ctl->steps + 1;
+  const int slot = L.ck_every > 0 ? after / L.ck_every : 0;
+  ctl->ck_slot = (L.ck_every > 0 && after % L.ck_every == 0 && slot < L.ck_cap &&
+                  ctl->rec_lo == ctl->rec_hi)
+                     ? slot
+                     : -1;
   if (L.dt_log && ctl->steps < L.dt_log_cap) L.dt_log[ctl->steps] = dt;
 }
 
@@ -200,6 +214,7 @@ __device__ inline void finish_step(LoopControl* ctl, const LoopConst& L, const T
   if (oi >= L.n_times || steps >= L.max_steps || ctl->nonfinite) {
     ctl->done = 1;
     ctl->rec_lo = ctl->rec_hi = 0;
+    ctl->ck_slot = -1;
   } else {
     plan_next_step(ctl, L, tot.inv, t_new, oi, rain_j);
   }

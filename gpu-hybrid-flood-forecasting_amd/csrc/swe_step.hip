@@ -53,6 +53,16 @@ __global__ __launch_bounds__(kBlock) void k_step(
     dtg = ctl->dtg_f;
     rec_lo = ctl->rec_lo;
     rec_hi = ctl->rec_hi;
+    // A checkpoint step (never also an output step) records into the
+    // checkpoint slots through the same stores.
+    const int ck = ctl->ck_slot;
+    if (ck >= 0) {
+      cube_h = L.ck_h;
+      cube_hu = L.ck_hu;
+      cube_hv = L.ck_hv;
+      rec_lo = ck;
+      rec_hi = ck + 1;
+    }
   }
   const int gw = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave_in_block();
   StripAcc acc;
@@ -143,6 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(Grid G, const float* __restri
       if (oi >= L.n_times || L.max_steps <= 0 || tot.nf) {
         ctl->done = 1;
         ctl->rec_lo = ctl->rec_hi = 0;
+        ctl->ck_slot = -1;
       } else {
         ctl->done = 0;
         plan_next_step(ctl, L, tot.inv, t0, oi, rj);
@@ -254,6 +265,8 @@ struct SweContext {
   int split_finish = 1;      // finish each loop step in its own one-workgroup kernel
   double* dt_log = nullptr;  // caller's device buffer for the dt schedule
   int dt_log_cap = 0;
+  float *ck_h = nullptr, *ck_hu = nullptr, *ck_hv = nullptr;  // caller's checkpoint cube
+  int ck_every = 0, ck_cap = 0;
   // swe_loop_begin/advance/end state
   double loop_times[2] = {0.0, 0.0};
   LoopConst loop_L{};
@@ -288,6 +301,11 @@ struct SweContext {
     L.times = d_times;
     L.dt_log = dt_log;
     L.dt_log_cap = dt_log_cap;
+    L.ck_h = ck_h;
+    L.ck_hu = ck_hu;
+    L.ck_hv = ck_hv;
+    L.ck_every = ck_every;
+    L.ck_cap = ck_cap;
     L.n_rain = (int)rain_t.size();
     L.n_times = n_times;
     L.max_steps = max_steps;
@@ -759,6 +777,19 @@ int swe_set_loop_finish(SweContext* c, int32_t split) {
     c->graph = nullptr;
     c->graph_key.clear();
   }
+  return SWE_OK;
+}
+
+int swe_set_checkpoints(SweContext* c, float* h, float* hu, float* hv, int32_t every,
+                        int32_t capacity) {
+  if (!c || every < 0 || capacity < 0 || ((every > 0 && capacity > 0) && (!h || !hu || !hv)))
+    return fail(SWE_EINVAL, "swe_set_checkpoints: bad arguments");
+  const bool on = every > 0 && capacity > 0;
+  c->ck_h = on ? h : nullptr;
+  c->ck_hu = on ? hu : nullptr;
+  c->ck_hv = on ? hv : nullptr;
+  c->ck_every = on ? every : 0;
+  c->ck_cap = on ? capacity : 0;
   return SWE_OK;
 }
 

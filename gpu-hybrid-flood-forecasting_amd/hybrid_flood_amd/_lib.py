@@ -95,6 +95,7 @@ SIGNATURES = {
     "swe_set_tile_rows": (c_int32, [c_void_p, c_int32]),
     "swe_set_waves_per_cu": (c_int32, [c_void_p, c_int32]),
     "swe_set_adjoint_rows": (c_int32, [c_void_p, c_int32]),
+    "swe_set_checkpoints": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32]),
     "swe_set_graph_chunk": (c_int32, [c_void_p, c_int32]),
     "swe_set_dt_log": (c_int32, [c_void_p, c_void_p, c_int32]),
     "swe_set_loop_finish": (c_int32, [c_void_p, c_int32]),

@@ -1,14 +1,17 @@
 """Checkpointed adjoint of the device time loop behind ``torch.autograd.Function``.
 
 The forward is the product path: ``integrate_to_outputs`` on the device,
-recording its float64 step sizes.  The schedule is then data: the reverse
-sweep replays steps with ``swe_step`` (bit-identical to the loop's steps,
-same kernels and the same float64 scalars), keeps one state every
-``C = ceil(sqrt(K))`` steps, and walks the segments backwards: each segment is
-recomputed from its checkpoint into a C-state buffer, then reversed step by
-step with the hand-written adjoint kernel (``swe_adjoint_sweep``, or
-``swe_adjoint_sweep_f64`` for float64 cotangents).  Memory is ~2 sqrt(K)
-states; compute is ~3 forward steps + 1 reverse step per step.
+recording its float64 step sizes and, every ``C = ceil(sqrt(max_steps))``
+steps, a checkpoint of the state (``swe_set_checkpoints``; the loop's own
+stores write it).  The schedule is then data: the reverse sweep walks the
+segments backwards, recomputes each from its checkpoint into a C-state
+buffer with ``swe_step`` (bit-identical to the loop's steps, same kernels and
+the same float64 scalars), then reverses it step by step with the
+hand-written adjoint kernel (``swe_adjoint_sweep``, or
+``swe_adjoint_sweep_f64`` for float64 cotangents).  Memory is ~2 sqrt(max_steps)
+states; compute is ~2 forward steps + 1 reverse step per step.  Without
+in-loop checkpoints the forward rebuilds them by one more replay, every
+ceil(sqrt(K)) steps.
 
 ``rollout_depths(manning_n, rain_multiplier, model)`` returns the depths at
 the model's output times and differentiates with respect to the per-cell
@@ -53,7 +56,8 @@ class RolloutModel:
     max_dt: float = 10.0
     max_steps: int = 20_000
     variant: str = "fast"
-    checkpoint_every: int = 0  # 0 = ceil(sqrt(K))
+    checkpoint_every: int = 0  # 0 = ceil(sqrt(max_steps)) recorded in the loop, else ceil(sqrt(K))
+    record_checkpoints: bool = True  # store checkpoints during the forward loop (no replay pass)
     adjoint_precision: str = "f32"  # cotangent arithmetic of the reverse sweep: "f32" or "f64"
 
     def params_for(self, manning_n: torch.Tensor, rain_multiplier) -> SWEParams:
@@ -82,18 +86,40 @@ def _clone(s: SWEState) -> SWEState:
 
 
 def rollout_forward(params: SWEParams, state0: SWEState, times, *, max_steps: int,
-                    variant: str, every: int = 0):
-    """Device-loop forward; returns the depth stack (T, ny, nx) and the tape."""
+                    variant: str, every: int = 0, record: bool = True):
+    """Device-loop forward; returns the depth stack (T, ny, nx) and the tape.
+
+    With ``record`` the loop itself stores a checkpoint every ``every`` steps
+    (default ceil(sqrt(max_steps)), since the step count is known only after
+    the run); a checkpoint that falls on an output step is taken from the
+    output cube.  Without it (or when the checkpoint storage does not fit in
+    device memory) the checkpoints are rebuilt afterwards by replaying the
+    recorded schedule, every ceil(sqrt(K)) steps.
+    """
     from .numerics import integrate_to_outputs
 
     ctx = context_for(params, variant)
     dev = state0.h.device
+    lib = _lib.load()
     log = torch.zeros(max(max_steps, 1), dtype=torch.float64, device=dev)
-    _lib.check(_lib.load().swe_set_dt_log(ctx.handle, log.data_ptr(), int(log.numel())), "swe_set_dt_log")
+    C = every if every > 0 else max(1, int(math.ceil(math.sqrt(max(max_steps, 1)))))
+    ck = None
+    if record:
+        try:
+            ck = torch.empty((3, max_steps // C + 1) + tuple(state0.h.shape), dtype=torch.float32,
+                             device=dev)
+        except torch.cuda.OutOfMemoryError:
+            ck = None
+    _lib.check(lib.swe_set_dt_log(ctx.handle, log.data_ptr(), int(log.numel())), "swe_set_dt_log")
+    if ck is not None:
+        _lib.check(lib.swe_set_checkpoints(ctx.handle, ck[0].data_ptr(), ck[1].data_ptr(),
+                                           ck[2].data_ptr(), C, int(ck.shape[1])),
+                   "swe_set_checkpoints")
     try:
         res = integrate_to_outputs(state0, params, times, max_steps=max_steps, variant=variant)
     finally:
-        _lib.load().swe_set_dt_log(ctx.handle, None, 0)
+        lib.swe_set_dt_log(ctx.handle, None, 0)
+        lib.swe_set_checkpoints(ctx.handle, None, None, None, 0, 0)
     K = int(res.steps_used)
     dts = log[:K].cpu().numpy().astype(np.float64)
     t64 = _output_times_f64(times)
@@ -113,16 +139,26 @@ def rollout_forward(params: SWEParams, state0: SWEState, times, *, max_steps: in
         while m < t64.shape[0] and clock[k] >= t64[m] - OUTPUT_SLACK_S:
             out_steps[m] = k
             m += 1
-    # Checkpoints by replay of the recorded schedule, one segment per call.
-    C = every if every > 0 else max(1, int(math.ceil(math.sqrt(max(K, 1)))))
     cps = {0: _clone(state0)}
-    lib = _lib.load()
-    s = _lib.stream_handle(dev)
-    for start in range(0, K - C, C) if K > C else ():
-        nxt = SWEState(*(torch.empty_like(x) for x in state0))
-        _lib.check(lib.swe_replay(ctx.handle, _lib.state_struct(*cps[start]), _dptr(dts, start),
-                                  _dptr(ts, start), C, _lib.state_struct(*nxt), s), "swe_replay")
-        cps[start + C] = nxt
+    if ck is not None:
+        # Slot s holds the state after s*C steps, unless an output was taken at
+        # that step: the output cube holds it then.
+        recorded = {int(k): i for i, k in enumerate(out_steps) if i >= 1}
+        for start in range(C, K, C):
+            i = recorded.get(start)
+            src = (res.states.h[i], res.states.hu[i], res.states.hv[i]) if i is not None else \
+                (ck[0, start // C], ck[1, start // C], ck[2, start // C])
+            cps[start] = SWEState(*(x.clone() for x in src))
+        del ck
+    else:
+        # Checkpoints by replay of the recorded schedule, one segment per call.
+        C = every if every > 0 else max(1, int(math.ceil(math.sqrt(max(K, 1)))))
+        s = _lib.stream_handle(dev)
+        for start in range(0, K - C, C) if K > C else ():
+            nxt = SWEState(*(torch.empty_like(x) for x in state0))
+            _lib.check(lib.swe_replay(ctx.handle, _lib.state_struct(*cps[start]), _dptr(dts, start),
+                                      _dptr(ts, start), C, _lib.state_struct(*nxt), s), "swe_replay")
+            cps[start + C] = nxt
     tape = Tape(params, dts, ts, out_steps, cps, C, K)
     return res.states.h, res, tape
 
@@ -225,7 +261,8 @@ class _RolloutFunction(torch.autograd.Function):
         params = model.params_for(manning_n, rain_multiplier)
         depths, res, tape = rollout_forward(params, model.h0, model.output_times_s,
                                             max_steps=model.max_steps, variant=model.variant,
-                                            every=model.checkpoint_every)
+                                            every=model.checkpoint_every,
+                                            record=model.record_checkpoints)
         ctx.tape = tape
         ctx.model = model
         ctx.res = res

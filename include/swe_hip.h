@@ -214,6 +214,16 @@ int swe_loop_end(SweContext* ctx, SweState state, SweRunInfo* info, void* stream
  * adjoint recomputes its checkpoints that way).  capacity 0 turns it off. */
 int swe_set_dt_log(SweContext* ctx, double* dt_log, int32_t capacity);
 
+/* Record checkpoints during the device loop: while set, the state after
+ * s * every steps (s = 1, 2, ...; s < capacity) is written to slot s of the
+ * caller's arrays h, hu, hv (each capacity x nx*ny floats), except when that
+ * step also writes an output-time slot of swe_run_to_outputs, whose cube then
+ * holds the same state.  every = 0 or capacity = 0 turns it off.  No reference
+ * counterpart: the reference differentiates with jax.grad; the checkpointed
+ * adjoint uses it to skip a replay of the forward. */
+int swe_set_checkpoints(SweContext* ctx, float* h, float* hu, float* hv, int32_t every,
+                        int32_t capacity);
+
 /* Use HIP graphs of `chunk` captured step launches for the device loop
  * (0 = plain launches).  Default 64. */
 int swe_set_graph_chunk(SweContext* ctx, int32_t chunk);

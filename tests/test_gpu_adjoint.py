@@ -369,3 +369,53 @@ def test_adjoint_rows_do_not_change_results(cuda, variant):
             np.testing.assert_array_equal(a, b)
     with pytest.raises(ValueError):
         ctx.set_adjoint_rows(-1)
+
+
+@pytest.mark.parametrize("every", [1, 7, 0])
+def test_loop_checkpoints_equal_replayed_checkpoints(cuda, every):
+    """Checkpoints stored by the device loop (incl. ones that fall on output
+    steps, every=1) are bit-identical to the ones rebuilt by replay, and so
+    are the gradients."""
+    import torch
+
+    from hybrid_flood_amd.adjoint import rollout_backward, rollout_forward
+
+    case, model = _rollout_setup(cuda)
+    params = model.params_for(torch.as_tensor(case["manning_n"], device=cuda), 1.0)
+    max_steps = 400 if every == 1 else model.max_steps
+    _, res_a, tape_a = rollout_forward(params, model.h0, model.output_times_s, max_steps=max_steps,
+                                       variant="fast", every=every, record=True)
+    c = tape_a.every
+    _, res_b, tape_b = rollout_forward(params, model.h0, model.output_times_s, max_steps=max_steps,
+                                       variant="fast", every=c, record=False)
+    assert tape_a.steps == tape_b.steps == res_a.steps_used > 10 and tape_b.every == c
+    assert sorted(tape_a.checkpoints) == sorted(tape_b.checkpoints) == list(range(0, tape_a.steps, c))
+    if every == 1:
+        assert set(int(k) for k in tape_a.out_steps[1:]) & set(tape_a.checkpoints)
+    for k in tape_a.checkpoints:
+        for x, y in zip(tape_a.checkpoints[k], tape_b.checkpoints[k]):
+            assert torch.equal(x, y), k
+    g = torch.as_tensor(np.random.default_rng(2).normal(size=tuple(res_a.states.h.shape)),
+                        dtype=torch.float32, device=cuda)
+    ga = rollout_backward(tape_a, g, variant="fast")
+    gb = rollout_backward(tape_b, g, variant="fast")
+    for x, y in zip(ga, gb):
+        for u, v in zip(x if isinstance(x, tuple) else (x,), y if isinstance(y, tuple) else (y,)):
+            assert torch.equal(u, v)
+
+
+def test_set_checkpoints_rejects_bad_arguments(cuda):
+    import torch
+
+    from hybrid_flood_amd import _lib
+    from hybrid_flood_amd import numerics as nm
+
+    case, model = _rollout_setup(cuda)
+    params = model.params_for(torch.as_tensor(case["manning_n"], device=cuda), 1.0)
+    ctx = nm.context_for(params, "fast")
+    lib = _lib.load()
+    with pytest.raises(ValueError):
+        _lib.check(lib.swe_set_checkpoints(ctx.handle, None, None, None, 5, 3), "swe_set_checkpoints")
+    with pytest.raises(ValueError):
+        _lib.check(lib.swe_set_checkpoints(ctx.handle, None, None, None, -1, 0), "swe_set_checkpoints")
+    _lib.check(lib.swe_set_checkpoints(ctx.handle, None, None, None, 0, 0), "swe_set_checkpoints")
