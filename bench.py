@@ -177,7 +177,8 @@ def bench_ensemble(variant: str, count: int, seed: int, t_end: float, plain_cups
     cups_full = total_cells * full_launches / (ms_full * 1e-3) if full_launches else None
     return {
         "workload": f"{count} urban members, sides from {{256,512,1024,2048}} (seed {seed}), "
-                    f"dx 2 m, {t_end:.0f} s each, per-member CFL dt, persistent graph-captured launches",
+                    f"dx 2 m, {t_end:.0f} s each, per-member CFL dt, graph-captured launches of one "
+                    "wavefront per strip over all running members",
         "members": count, "cells": int(sum(m.params.bed.numel() for m in members)),
         "member_sides": {str(k): sides.count(k) for k in sorted(set(sides))},
         "launches": launches, "member_steps_total": int(sum(i["steps_used"] for i in infos)),
@@ -187,7 +188,7 @@ def bench_ensemble(variant: str, count: int, seed: int, t_end: float, plain_cups
                                 "cell_updates_per_s": cups_full,
                                 "vs_plain_step": (cups_full / plain_cups) if cups_full else None},
         "max_member_steps": int(max(i["steps_used"] for i in infos)),
-        "rows": tune["rows"], "status": rc,
+        "rows": tune["rows"], "tune_ms_per_launch": tune.get("candidates"), "status": rc,
         "max_abs_mass_balance_rel": max(abs(i["mass_balance_rel"]) for i in infos),
     }
 

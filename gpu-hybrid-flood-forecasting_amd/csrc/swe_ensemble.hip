@@ -1,10 +1,14 @@
 // Batched ensemble of independent shallow-water domains of uneven size.
 //
 // Every launch of k_ensemble_step advances each unfinished member by one of
-// its own adaptive steps.  The launch is persistent: a fixed grid sized to
-// the resident capacity of the device walks the flattened list of all
-// members' strips (62 columns x `rows` rows each) with a grid stride, so
-// members of 256^2 and 2048^2 cells share one launch and one load balance.
+// its own adaptive steps.  One launch covers the flattened list of all
+// members' strips (62 columns x `rows` rows each), one wavefront per strip,
+// so members of 256^2 and 2048^2 cells share one launch and the hardware
+// dispatcher balances them.  (A persistent grid sized to the resident
+// capacity, each wavefront walking several strips, measured 18 % slower on
+// MI355X: every strip then starts with a serial chain of member lookup and
+// control loads that no other wavefront hides.)  Like the single-domain step,
+// the launch reserves LDS to hold 16 wavefronts per compute unit.
 // Each member keeps its own control block (dt, clock, step count) and its
 // own per-wavefront partials, written with plain stores; after the launch,
 // k_ensemble_finish (one workgroup per running member) reduces each
@@ -74,9 +78,20 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
                                                       int n_members, int total_waves,
                                                       EnsPartials P) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int stride = gridDim.x * (blockDim.x / kWave);
-  for (int w = blockIdx.x * (blockDim.x / kWave) + wave_in_block(); w < total_waves;
-       w += stride) {
+  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so the
+  // workgroups of XCD x (blockIdx % 8 == x) take the x-th contiguous eighth
+  // of the flattened strip list.  Neighbouring strips then run on one XCD and
+  // share their halo rows through its L2.  The host makes gridDim a multiple
+  // of 8 and large enough for one strip per wavefront; the stride loop keeps
+  // any smaller grid correct.  Every strip belongs to exactly one chunk.
+  constexpr int kXcd = 8;
+  const int wpb = blockDim.x / kWave;
+  const int per_xcd = gridDim.x / kXcd;
+  const int chunk = (total_waves + kXcd - 1) / kXcd;
+  const int w_lo = (blockIdx.x % kXcd) * chunk;
+  const int w_hi = min(w_lo + chunk, total_waves);
+  const int stride = per_xcd * wpb;
+  for (int w = w_lo + (blockIdx.x / kXcd) * wpb + wave_in_block(); w < w_hi; w += stride) {
     const int m = find_member(mem, n_members, w);
     const Member& mb = mem[m];
     const LoopControl* ctl = mb.ctl;
@@ -155,6 +170,7 @@ struct SweEnsemble {
   int parts_cap = 0;
   int total_waves = 0;
   int grid = 0;
+  unsigned step_lds = 0;  // dynamic LDS per step workgroup: caps residency at 16 waves/CU
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t graph = nullptr;
   double times_host[2] = {0.0, 0.0};
@@ -176,7 +192,8 @@ int layout(SweEnsemble* e, const std::vector<char>* running = nullptr) {
     ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<RefMath>, 256, 0));
   else
     ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<FastMath>, 256, 0));
-  const long slots = (long)cus * std::max(per_cu, 1) * 4;  // resident wavefronts
+  // Resident wavefronts: at most 4 workgroups (16 wavefronts) per CU under the LDS cap.
+  const long slots = (long)cus * std::min(std::max(per_cu, 1), 4) * 4;
   auto waves_for = [&](int rows) {
     long w = 0;
     for (size_t m = 0; m < e->host.size(); ++m) {
@@ -218,7 +235,14 @@ int layout(SweEnsemble* e, const std::vector<char>* running = nullptr) {
   }
   if (!e->act.empty())
     ENS_TRY(hipMemcpy(e->d_mem, e->act.data(), sizeof(Member) * e->act.size(), hipMemcpyHostToDevice));
-  e->grid = std::max(1, (int)std::min<long>(slots / 4, ((long)base + 3) / 4));
+  // One wavefront per strip, in a multiple of 8 workgroups (one share per
+  // XCD, see k_ensemble_step).
+  const long want = ((long)base + 3) / 4;
+  e->grid = (int)std::max<long>(8, (want + 7) / 8 * 8);
+  int lds_cu = 0;
+  ENS_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+  const unsigned lds = lds_cu > 0 ? (unsigned)(lds_cu / 5 + 512) : 0u;  // 4 workgroups fit, 5 do not
+  e->step_lds = (lds > 0 && lds <= 65536 && 4L * lds <= (long)lds_cu - 1024) ? lds : 0u;
   if (e->graph) {
     (void)hipGraphExecDestroy(e->graph);
     e->graph = nullptr;
@@ -228,10 +252,10 @@ int layout(SweEnsemble* e, const std::vector<char>* running = nullptr) {
 
 void launch_ens(SweEnsemble* e, hipStream_t s) {
   if (e->variant == SWE_VARIANT_REFERENCE)
-    hipLaunchKernelGGL(k_ensemble_step<RefMath>, dim3(e->grid), dim3(256), 0, s, e->d_mem,
+    hipLaunchKernelGGL(k_ensemble_step<RefMath>, dim3(e->grid), dim3(256), e->step_lds, s, e->d_mem,
                        (int)e->act.size(), e->total_waves, e->P);
   else
-    hipLaunchKernelGGL(k_ensemble_step<FastMath>, dim3(e->grid), dim3(256), 0, s, e->d_mem,
+    hipLaunchKernelGGL(k_ensemble_step<FastMath>, dim3(e->grid), dim3(256), e->step_lds, s, e->d_mem,
                        (int)e->act.size(), e->total_waves, e->P);
   hipLaunchKernelGGL(k_ensemble_finish, dim3((unsigned)e->act.size()), dim3(256), 0, s, e->d_mem,
                      e->P);

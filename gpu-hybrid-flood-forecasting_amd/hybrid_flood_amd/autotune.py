@@ -111,7 +111,7 @@ def tune_rows(params: SWEParams, state: SWEState, *, variant: str | None = None,
 
 
 def tune_ensemble_rows(runner, *, t_end: float = 1.0e9, launches: int = 16,
-                       candidates=(8, 16, 32), use_cache: bool = True) -> dict:
+                       candidates=(8, 12, 16, 20, 24, 32, 40, 48), use_cache: bool = True) -> dict:
     """Pick the strip height for an ensemble by timing whole ensemble launches."""
     shapes = sorted(tuple(m.params.bed.shape) for m in runner.members)
     key = _key("ensemble", (len(shapes), sum(a * b for a, b in shapes)), runner.variant)

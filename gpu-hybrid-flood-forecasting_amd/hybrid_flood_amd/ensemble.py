@@ -2,7 +2,7 @@
 
 Each member is a full domain (its own DEM, Manning field, mask, rain series,
 boundary codes and controls).  ``EnsembleRunner.run`` advances all members to
-``t_end`` in persistent, graph-captured launches; every launch takes one CFL
+``t_end`` in graph-captured launches; every launch takes one CFL
 step of every unfinished member, with each member's own dt.  The per-member
 arithmetic is the single-domain fused step, so a member's result is identical
 to running it alone with ``integrate_to_outputs(state, params, [0, t_end])``.

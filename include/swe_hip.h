@@ -278,9 +278,9 @@ int swe_adjoint_sweep_f64(SweContext* ctx, const float* states_h, const float* s
  * Batched ensemble (no counterpart in the reference, which runs one domain;
  * it runs many independent rainfall/roughness scenarios of the same step).
  * Each member is a full domain of its own size with its own fields, rain
- * series and controls.  One persistent launch advances every unfinished
- * member by one of its own CFL steps; chunks of launches are captured in a
- * HIP graph.
+ * series and controls.  One launch (one wavefront per strip of every
+ * running member) advances every unfinished member by one of its own CFL
+ * steps; chunks of launches are captured in a HIP graph.
  * ------------------------------------------------------------------------ */
 typedef struct SweEnsemble SweEnsemble;
 
@@ -312,7 +312,7 @@ int swe_ensemble_end(SweEnsemble* ens, SweRunInfo* infos, void* stream);
 /* begin + advance/poll until every member reaches t_end + end. */
 int swe_ensemble_run(SweEnsemble* ens, double t_end, int32_t max_steps, SweRunInfo* infos,
                      void* stream);
-/* Strips (wavefronts of work) per launch and the persistent grid size. */
+/* Strips (wavefronts of work) per launch and the launch's workgroup count. */
 int swe_ensemble_total_waves(const SweEnsemble* ens, int32_t* waves, int32_t* grid);
 
 /* ---- Hybrid coupled forecast (hybrid/coupled_forecast.py) ---------------- */
