@@ -17,6 +17,8 @@
 // rest, wall mass flux and open-boundary accounting keep their exact forms.
 #pragma once
 
+#include <type_traits>
+
 #include "swe_device.hpp"
 #include "swe_strip.hpp"
 
@@ -130,6 +132,7 @@ struct RawCell {
   float h, hu, hv, b;
   unsigned a;
   bool neg_hv;  // row beyond a reflective south/north edge
+  float n, m;   // Manning n and rain multiplier (packed layout only)
 };
 
 // Per-strip addressing through raw buffer resources.  Each array gets a
@@ -179,8 +182,58 @@ __device__ __forceinline__ StripIO strip_io(const Grid& G, const float* H, const
   return io;
 }
 
+// Packed layout of the fast device loop: 12 bytes per cell for the work
+// state (h, hu, hv) and 12 for the static fields (bed, n', multiplier), where
+// n' = |n| on an active cell and -1 on an inactive one -- the friction law
+// uses n^2 only, and the sign carries the domain mask.  A row costs two
+// 96-bit loads per lane instead of seven scalar ones, and the step streams
+// three arrays through HBM instead of ten.
+// The packed rows go through plain global pointers: three adjacent floats per
+// lane compile to one global_load_dwordx3 / global_store_dwordx3.  (The raw
+// buffer forms are not usable here with the ROCm 7.2 compiler: element reads
+// after __builtin_bit_cast of the 96-bit load builtin are miscompiled into
+// one 32-bit load standing for all three elements; the whole-vector form is
+// split back into 32-bit loads; the volatile form is made system-coherent,
+// which bypasses the L2 that neighbouring strips share.)
+struct PackedIO {
+  const float* st;  // packed state
+  const float* sx;  // packed static fields
+  unsigned off12;   // this lane's column, in bytes, in a packed row
+  size_t row3;      // floats per packed row (3 nx)
+};
+
+__device__ __forceinline__ PackedIO packed_io(const Grid& G, const float* S, int ic) {
+  PackedIO io;
+  io.st = S;
+  io.sx = G.pk;
+  io.off12 = (unsigned)ic * 12u;
+  io.row3 = 3 * (size_t)G.nx;
+  return io;
+}
+
+__device__ __forceinline__ const float* packed_at(const float* base, size_t row3, int jj,
+                                                  unsigned off12) {
+  return (const float*)((const char*)(base + (size_t)jj * row3) + off12);
+}
+
 // Row j of the lane's column: clamped row index (numerics.py:29-55); the
 // reflective-edge sign flips are applied in side_of.
+__device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, int j) {
+  const int jj = min(max(j, 0), G.ny - 1);
+  const float* st = packed_at(io.st, io.row3, jj, io.off12);
+  const float* sx = packed_at(io.sx, io.row3, jj, io.off12);
+  RawCell r;
+  r.h = st[0];
+  r.hu = st[1];
+  r.hv = st[2];
+  r.b = sx[0];
+  r.n = sx[1];
+  r.m = sx[2];
+  r.a = signbit(r.n) ? 0u : 1u;  // sign bit clear: active
+  r.neg_hv = (j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0);
+  return r;
+}
+
 __device__ __forceinline__ RawCell load_raw(const Grid& G, const StripIO& io, int j) {
   const int jj = min(max(j, 0), G.ny - 1);
   RawCell r;
@@ -212,8 +265,10 @@ __device__ __forceinline__ RowCell side_of(const RawCell& r, bool neg_hu, float 
 
 // One wavefront's strip: columns sx*62-1 .. sx*62+62, rows sy*rows ..
 // sy*rows+rows-1.  sx, sy and rows must be wave-uniform.  The raw inputs of
-// row j+2 are in flight while row j is computed.
-template <bool LOOP>
+// row j+2 are in flight while row j is computed.  PACKED: H and Ho are packed
+// states (3 floats per cell, HU/HV unused) and the static fields come from
+// G.pk; the arithmetic is the same, so both layouts give identical bits.
+template <bool LOOP, bool PACKED = false>
 __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
                                                  const float* __restrict__ HV,
@@ -235,18 +290,31 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
   const int j0 = sy * rows;
   const int j1 = min(j0 + rows, G.ny);
   const float rdx = G.rdx, rdy = G.rdy, dry = G.dry;
-  const StripIO io = strip_io(G, H, HU, HV, max(j0 - 1, 0), ic);
+  using IO = std::conditional_t<PACKED, PackedIO, StripIO>;
+  IO io;
+  if constexpr (PACKED)
+    io = packed_io(G, H, ic);
+  else
+    io = strip_io(G, H, HU, HV, max(j0 - 1, 0), ic);
   const size_t out0 = (size_t)j0 * (size_t)G.nx, out_cells = (size_t)(j1 - j0) * (size_t)G.nx;
-  const Rsrc oh = make_rsrc(Ho + out0, out_cells * 4);
-  const Rsrc ohu = make_rsrc(HUo + out0, out_cells * 4);
-  const Rsrc ohv = make_rsrc(HVo + out0, out_cells * 4);
+  Rsrc oh, ohu, ohv;
+  if constexpr (!PACKED) {
+    oh = make_rsrc(Ho + out0, out_cells * 4);
+    ohu = make_rsrc(HUo + out0, out_cells * 4);
+    ohv = make_rsrc(HVo + out0, out_cells * 4);
+  }
 
   // Prologue: the south face of row j0, rows j0 (current) and j0+1 (in flight).
   RowCell cur = side_of(load_raw(G, io, j0 - 1), neg_hu, dry);
   RawCell next = load_raw(G, io, j0);
   float yt0, yt1, yt2;
+  float nn, mult;
   {
     const RowCell c0 = side_of(next, neg_hu, dry);
+    if constexpr (PACKED) {
+      nn = next.n;
+      mult = next.m;
+    }
     next = load_raw(G, io, j0 + 1);
     const FaceFlux fy = face_fast_any<false>(G, cur.s, c0.s, __all(cur.s.a && c0.s.a));
     if (LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
@@ -255,8 +323,10 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
     yt2 = fy.fn + fy.cB;
     cur = c0;
   }
-  float nn = io.ld(io.n, j0);
-  float mult = io.ld(io.mult, j0);
+  if constexpr (!PACKED) {
+    nn = io.ld(io.n, j0);
+    mult = io.ld(io.mult, j0);
+  }
   float nf_probe = 0.0f;
   float clip32 = 0.0f;  // clip depth is small; float32 per lane over the strip
 
@@ -268,8 +338,13 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
     // Unconditional, so the wait counts stay exact; past the strip the
     // clamped rows were just loaded and come from the cache.
     ahead = load_raw(G, io, min(j + 2, j1));
-    nn_next = io.ld(io.n, min(j + 1, j1 - 1));
-    mult_next = io.ld(io.mult, min(j + 1, j1 - 1));
+    if constexpr (PACKED) {  // row j+1's fields arrived with its state
+      nn_next = north.n;
+      mult_next = north.m;
+    } else {
+      nn_next = io.ld(io.n, min(j + 1, j1 - 1));
+      mult_next = io.ld(io.mult, min(j + 1, j1 - 1));
+    }
 
     SideF w;
     w.h = from_west(cur.s.h);
@@ -311,12 +386,20 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
     const float hu_o = wet ? hus * rD : 0.0f;
     const float hv_o = wet ? hvs * rD : 0.0f;
     if (own) {
-      const unsigned so = (unsigned)(j - j0) * io.pitch;
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, h_o), oh, io.off4, so, 0);
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, hu_o), ohu, io.off4, so,
-                                            0);
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, hv_o), ohv, io.off4, so,
-                                            0);
+      if constexpr (PACKED) {
+        float* o = const_cast<float*>(packed_at(Ho, io.row3, j, io.off12));
+        o[0] = h_o;
+        o[1] = hu_o;
+        o[2] = hv_o;
+      } else {
+        const unsigned so = (unsigned)(j - j0) * io.pitch;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, h_o), oh, io.off4, so,
+                                              0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, hu_o), ohu, io.off4,
+                                              so, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, hv_o), ohv, io.off4,
+                                              so, 0);
+      }
       if (LOOP) {
         const size_t idx = (size_t)j * (size_t)G.nx + (size_t)ic;
         for (int k = rec_lo; k < rec_hi; ++k) {

@@ -37,7 +37,7 @@ namespace swe {
 // ---------------------------------------------------------------------------
 // The fused step
 // ---------------------------------------------------------------------------
-template <class M, int MODE>
+template <class M, int MODE, bool PACKED = false>
 __global__ __launch_bounds__(kBlock) void k_step(
     Grid G, const float* __restrict__ H, const float* __restrict__ HU,
     const float* __restrict__ HV, float* __restrict__ Ho, float* __restrict__ HUo,
@@ -64,13 +64,14 @@ __global__ __launch_bounds__(kBlock) void k_step(
       rec_hi = ck + 1;
     }
   }
+  static_assert(!PACKED || (M::kFast && MODE != MODE_STEP), "packed layout: fast device loop only");
   const int gw = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave_in_block();
   StripAcc acc;
   if (gw < n_waves) {
     if constexpr (M::kFast)
-      march_strip_fast<MODE != MODE_STEP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
-                                          rows, dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
-                                          cube_h, cube_hu, cube_hv, acc);
+      march_strip_fast<MODE != MODE_STEP, PACKED>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
+                                                  gw / segs_x, rows, dt, dtrate, dtg, L.dxd, L.dyd,
+                                                  rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
     else
       march_strip<M, MODE != MODE_STEP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
                                         rows, dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
@@ -198,6 +199,45 @@ __global__ __launch_bounds__(kBlock) void k_fill3(float* __restrict__ a, float* 
   }
 }
 
+// Packed layout of the fast device loop (see swe_fast.hpp): static fields and
+// state in, state out.
+__global__ __launch_bounds__(kBlock) void k_pack_static(Grid G, float* __restrict__ pk, size_t N) {
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    pk[3 * k] = G.bed[k];
+    pk[3 * k + 1] = G.mask[k] ? fabsf(G.n[k]) : -1.0f;
+    pk[3 * k + 2] = G.mult[k];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_state(const float* __restrict__ h,
+                                                      const float* __restrict__ hu,
+                                                      const float* __restrict__ hv,
+                                                      float* __restrict__ S, size_t N) {
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    S[3 * k] = h[k];
+    S[3 * k + 1] = hu[k];
+    S[3 * k + 2] = hv[k];
+  }
+}
+
+// ctl non-null: the source is a or b by the parity of the loop's step count.
+__global__ __launch_bounds__(kBlock) void k_unpack_state(const LoopControl* __restrict__ ctl,
+                                                        const float* __restrict__ a,
+                                                        const float* __restrict__ b,
+                                                        float* __restrict__ h,
+                                                        float* __restrict__ hu,
+                                                        float* __restrict__ hv, size_t N) {
+  const float* S = (ctl && (ctl->steps & 1)) ? b : a;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    h[k] = S[3 * k];
+    hu[k] = S[3 * k + 1];
+    hv[k] = S[3 * k + 2];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy_final(const LoopControl* __restrict__ ctl,
                                                       const float* __restrict__ a,
                                                       const float* __restrict__ b,
@@ -258,7 +298,9 @@ struct SweContext {
   LoopControl* h_ctl = nullptr;  // pinned mirror
   Partials P{};
   int parts_cap = 0;
-  float* work = nullptr;  // 2 x 3 x N ping-pong state
+  float* work = nullptr;  // 2 x 3 x N ping-pong state (SoA, or packed: see ws())
+  float* pk = nullptr;    // packed static fields of the fast device loop (3 x N)
+  bool loop_packed = false;  // layout of the work state of the current device loop
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t graph = nullptr;
   std::vector<const void*> graph_key;
@@ -279,6 +321,10 @@ struct SweContext {
 
   size_t cells() const { return (size_t)G.nx * (size_t)G.ny; }
   float* buf(int which, int field) { return work + ((size_t)which * 3 + field) * cells(); }
+  // Packed work state `which`: (h, hu, hv) interleaved, in the same memory.
+  float* ws(int which) { return work + (size_t)which * 3 * cells(); }
+  // The fast variant's device loop runs on the packed layout.
+  bool packed_variant() const { return variant == SWE_VARIANT_FAST; }
   int segs_x() const { return (G.nx + kStripCols - 1) / kStripCols; }
   int n_waves() const { return segs_x() * ((G.ny + rows - 1) / rows); }
   int step_blocks() const { return (n_waves() + kWavesPerBlock - 1) / kWavesPerBlock; }
@@ -346,19 +392,49 @@ int ensure_times(SweContext* c, int n) {
   return SWE_OK;
 }
 
+int copy_state(SweContext* c, float* dh, float* dhu, float* dhv, const float* sh,
+               const float* shu, const float* shv, hipStream_t s);
+
 int ensure_work(SweContext* c) {
-  if (c->work) return SWE_OK;
-  SWE_HIP_TRY(hipMalloc(&c->work, sizeof(float) * 6 * c->cells()));
+  if (!c->work) SWE_HIP_TRY(hipMalloc(&c->work, sizeof(float) * 6 * c->cells()));
+  if (!c->pk) {
+    SWE_HIP_TRY(hipMalloc(&c->pk, sizeof(float) * 3 * c->cells()));
+    c->G.pk = c->pk;
+  }
+  return SWE_OK;
+}
+
+// Start a device loop on `src`: pick the work layout for the variant and fill
+// work state 0 (for the packed layout also the packed static fields, from the
+// caller's current arrays).
+int stage_in(SweContext* c, const SweState& src, hipStream_t s) {
+  c->loop_packed = c->packed_variant();
+  if (!c->loop_packed) return copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), src.h, src.hu, src.hv, s);
+  const size_t N = c->cells();
+  const int nb = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL(k_pack_static, dim3(nb), dim3(kBlock), 0, s, c->G, c->pk, N);
+  hipLaunchKernelGGL(k_pack_state, dim3(nb), dim3(kBlock), 0, s, src.h, src.hu, src.hv, c->ws(0), N);
+  SWE_HIP_TRY(hipGetLastError());
+  return SWE_OK;
+}
+
+// Copy work state `which` of the current loop to the caller's arrays.
+int stage_out(SweContext* c, int which, float* dh, float* dhu, float* dhv, hipStream_t s) {
+  if (!c->loop_packed) return copy_state(c, dh, dhu, dhv, c->buf(which, 0), c->buf(which, 1), c->buf(which, 2), s);
+  const size_t N = c->cells();
+  const int nb = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL(k_unpack_state, dim3(nb), dim3(kBlock), 0, s, nullptr, c->ws(which), c->ws(which), dh, dhu, dhv, N);
+  SWE_HIP_TRY(hipGetLastError());
   return SWE_OK;
 }
 
 bool valid_state(const SweState& s) { return s.h && s.hu && s.hv; }
 
-template <class M, int MODE>
+template <class M, int MODE, bool PACKED = false>
 void launch_step(SweContext* c, const float* h, const float* hu, const float* hv, float* ho,
                  float* huo, float* hvo, float dt, float dtrate, float dtg, const LoopConst& L,
                  float* ch, float* chu, float* chv, hipStream_t s) {
-  hipLaunchKernelGGL((k_step<M, MODE>), dim3(c->step_blocks()), dim3(kBlock), c->step_lds, s, c->G, h, hu,
+  hipLaunchKernelGGL((k_step<M, MODE, PACKED>), dim3(c->step_blocks()), dim3(kBlock), c->step_lds, s, c->G, h, hu,
                      hv, ho, huo, hvo, dt, dtrate, dtg, c->rows, c->segs_x(), c->n_waves(),
                      c->d_ctl, L, c->P, ch, chu, chv);
 }
@@ -367,7 +443,10 @@ void launch_step_any(SweContext* c, int mode, const float* h, const float* hu, c
                      float* ho, float* huo, float* hvo, float dt, float dtrate, float dtg,
                      const LoopConst& L, float* ch, float* chu, float* chv, hipStream_t s) {
   if (mode == MODE_LOOP && c->split_finish) {
-    if (c->variant == SWE_VARIANT_REFERENCE)
+    if (c->loop_packed)
+      launch_step<FastMath, MODE_LOOP_SPLIT, true>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L,
+                                                   ch, chu, chv, s);
+    else if (c->variant == SWE_VARIANT_REFERENCE)
       launch_step<RefMath, MODE_LOOP_SPLIT>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch,
                                             chu, chv, s);
     else
@@ -385,7 +464,10 @@ void launch_step_any(SweContext* c, int mode, const float* h, const float* hu, c
       launch_step<RefMath, MODE_STEP>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch, chu,
                                       chv, s);
   } else {
-    if (mode == MODE_LOOP)
+    if (mode == MODE_LOOP && c->loop_packed)
+      launch_step<FastMath, MODE_LOOP, true>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch,
+                                             chu, chv, s);
+    else if (mode == MODE_LOOP)
       launch_step<FastMath, MODE_LOOP>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch, chu,
                                        chv, s);
     else
@@ -426,6 +508,18 @@ double host_interp(const SweContext* c, double t) {
   return interp_at(c->rain_t.data(), c->rain_r.data(), n, c->rain_default, t, j);
 }
 
+// One device-loop step reading work state a and writing 1 - a, in the layout
+// of the current loop.
+void launch_loop_step(SweContext* c, int a, const LoopConst& L, float* ch, float* chu, float* chv,
+                      hipStream_t s) {
+  if (c->loop_packed)
+    launch_step_any(c, MODE_LOOP, c->ws(a), nullptr, nullptr, c->ws(1 - a), nullptr, nullptr, 0.f,
+                    0.f, 0.f, L, ch, chu, chv, s);
+  else
+    launch_step_any(c, MODE_LOOP, c->buf(a, 0), c->buf(a, 1), c->buf(a, 2), c->buf(1 - a, 0),
+                    c->buf(1 - a, 1), c->buf(1 - a, 2), 0.f, 0.f, 0.f, L, ch, chu, chv, s);
+}
+
 // Launch `count` loop steps starting at launch parity 0 (buffer A -> B first).
 int launch_loop_chunk(SweContext* c, int count, const LoopConst& L, float* ch, float* chu,
                       float* chv, hipStream_t s) {
@@ -436,7 +530,8 @@ int launch_loop_chunk(SweContext* c, int count, const LoopConst& L, float* ch, f
                                     (const void*)(intptr_t)L.n_times,
                                     (const void*)(intptr_t)L.max_steps,
                                     (const void*)(intptr_t)c->split_finish,
-                                    (const void*)(intptr_t)c->step_lds};
+                                    (const void*)(intptr_t)c->step_lds,
+                                    (const void*)(intptr_t)c->loop_packed};
     // The loop constants are baked into the captured kernel arguments.
     uintptr_t words[(sizeof(LoopConst) + sizeof(uintptr_t) - 1) / sizeof(uintptr_t)] = {};
     std::memcpy(words, &L, sizeof(LoopConst));
@@ -448,12 +543,7 @@ int launch_loop_chunk(SweContext* c, int count, const LoopConst& L, float* ch, f
       }
       if (!c->cap_stream) SWE_HIP_TRY(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
       SWE_HIP_TRY(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < c->graph_chunk; ++k) {
-        const int a = k & 1;
-        launch_step_any(c, MODE_LOOP, c->buf(a, 0), c->buf(a, 1), c->buf(a, 2), c->buf(1 - a, 0),
-                        c->buf(1 - a, 1), c->buf(1 - a, 2), 0.f, 0.f, 0.f, L, ch, chu, chv,
-                        c->cap_stream);
-      }
+      for (int k = 0; k < c->graph_chunk; ++k) launch_loop_step(c, k & 1, L, ch, chu, chv, c->cap_stream);
       hipGraph_t g = nullptr;
       SWE_HIP_TRY(hipStreamEndCapture(c->cap_stream, &g));
       hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
@@ -464,11 +554,7 @@ int launch_loop_chunk(SweContext* c, int count, const LoopConst& L, float* ch, f
     for (int done = 0; done < count; done += c->graph_chunk) SWE_HIP_TRY(hipGraphLaunch(c->graph, s));
     return SWE_OK;
   }
-  for (int k = 0; k < count; ++k) {
-    const int a = k & 1;
-    launch_step_any(c, MODE_LOOP, c->buf(a, 0), c->buf(a, 1), c->buf(a, 2), c->buf(1 - a, 0),
-                    c->buf(1 - a, 1), c->buf(1 - a, 2), 0.f, 0.f, 0.f, L, ch, chu, chv, s);
-  }
+  for (int k = 0; k < count; ++k) launch_loop_step(c, k & 1, L, ch, chu, chv, s);
   SWE_HIP_TRY(hipGetLastError());
   return SWE_OK;
 }
@@ -530,8 +616,12 @@ int drive_loop(SweContext* c, const LoopConst& L, float* ch, float* chu, float* 
 int copy_final(SweContext* c, SweState dst, hipStream_t s) {
   const size_t N = c->cells();
   const int nb = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 4096);
-  hipLaunchKernelGGL(k_copy_final, dim3(nb), dim3(kBlock), 0, s, c->d_ctl, c->buf(0, 0),
-                     c->buf(1, 0), dst.h, dst.hu, dst.hv, N);
+  if (c->loop_packed)
+    hipLaunchKernelGGL(k_unpack_state, dim3(nb), dim3(kBlock), 0, s, c->d_ctl, c->ws(0), c->ws(1),
+                       dst.h, dst.hu, dst.hv, N);
+  else
+    hipLaunchKernelGGL(k_copy_final, dim3(nb), dim3(kBlock), 0, s, c->d_ctl, c->buf(0, 0),
+                       c->buf(1, 0), dst.h, dst.hu, dst.hv, N);
   SWE_HIP_TRY(hipGetLastError());
   return SWE_OK;
 }
@@ -711,6 +801,7 @@ int swe_destroy(SweContext* c) {
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->work) (void)hipFree(c->work);
+  if (c->pk) (void)hipFree(c->pk);
   if (c->P.vol) (void)hipFree(c->P.vol);
   if (c->d_times) (void)hipFree(c->d_times);
   if (c->d_ctl) (void)hipFree(c->d_ctl);
@@ -725,6 +816,7 @@ int swe_set_variant(SweContext* c, int32_t variant) {
   if (!c) return fail(SWE_EINVAL, "swe_set_variant: null context");
   if (variant != SWE_VARIANT_REFERENCE && variant != SWE_VARIANT_FAST)
     return fail(SWE_EINVAL, "swe_set_variant: unknown kernel variant");
+  if (c->loop_active) return fail(SWE_EINVAL, "swe_set_variant: a device loop is active");
   c->variant = variant;
   return SWE_OK;
 }
@@ -870,12 +962,10 @@ int swe_run_to_outputs(SweContext* c, SweState initial, const double* times, int
   if ((rc = ensure_work(c)) || (rc = ensure_partials(c)) || (rc = ensure_times(c, n_times))) return rc;
   SWE_HIP_TRY(hipMemcpyAsync(c->d_times, times, sizeof(double) * n_times, hipMemcpyHostToDevice, s));
   const size_t N = c->cells();
-  if ((rc = copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), initial.h, initial.hu,
-                       initial.hv, s)))
-    return rc;
+  if ((rc = stage_in(c, initial, s))) return rc;
   if ((rc = copy_state(c, out_h, out_hu, out_hv, initial.h, initial.hu, initial.hv, s))) return rc;
   const LoopConst L = c->loop_const(n_times, max_steps);
-  launch_reduce(c, RED_INIT, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), L, s);
+  launch_reduce(c, RED_INIT, initial.h, initial.hu, initial.hv, L, s);
   SWE_HIP_TRY(hipGetLastError());
   // Targets already met at t0 hold the initial state (no step is taken for them).
   int pre = 1;
@@ -888,16 +978,13 @@ int swe_run_to_outputs(SweContext* c, SweState initial, const double* times, int
   if ((rc = drive_loop(c, L, out_h, out_hu, out_hv, s, max_steps))) return rc;
   const LoopControl k = c->h_ctl[0];
   const int fin = k.steps & 1;  // launch n reads buffer n&1 and writes the other
-  float* fh = c->buf(fin, 0);
-  float* fhu = c->buf(fin, 1);
-  float* fhv = c->buf(fin, 2);
   for (int slot = k.out_idx; slot < n_times; ++slot) {
-    if ((rc = copy_state(c, out_h + (size_t)slot * N, out_hu + (size_t)slot * N,
-                         out_hv + (size_t)slot * N, fh, fhu, fhv, s)))
+    if ((rc = stage_out(c, fin, out_h + (size_t)slot * N, out_hu + (size_t)slot * N,
+                        out_hv + (size_t)slot * N, s)))
       return rc;
   }
   if (valid_state(final_state)) {
-    if ((rc = copy_state(c, final_state.h, final_state.hu, final_state.hv, fh, fhu, fhv, s))) return rc;
+    if ((rc = stage_out(c, fin, final_state.h, final_state.hu, final_state.hv, s))) return rc;
   }
   SWE_HIP_TRY(hipStreamSynchronize(s));
   fill_info(c, k, info);
@@ -926,19 +1013,17 @@ int swe_advance_to_time(SweContext* c, SweState in, double t_start, double t_tar
   if ((rc = ensure_work(c)) || (rc = ensure_partials(c)) || (rc = ensure_times(c, 2))) return rc;
   double times[2] = {t_start, t_target};
   SWE_HIP_TRY(hipMemcpyAsync(c->d_times, times, sizeof times, hipMemcpyHostToDevice, s));
-  if ((rc = copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), in.h, in.hu, in.hv, s)))
-    return rc;
+  if ((rc = stage_in(c, in, s))) return rc;
   LoopConst L = c->loop_const(2, max_substeps);
   L.record = 0;
-  launch_reduce(c, RED_INIT, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), L, s);
+  launch_reduce(c, RED_INIT, in.h, in.hu, in.hv, L, s);
   SWE_HIP_TRY(hipGetLastError());
   // The time arguments stay alive until the copy above has run.
   SWE_HIP_TRY(hipStreamSynchronize(s));
   if ((rc = drive_loop(c, L, nullptr, nullptr, nullptr, s, max_substeps))) return rc;
   const LoopControl k = c->h_ctl[0];
   const int fin = k.steps & 1;  // launch n reads buffer n&1 and writes the other
-  if ((rc = copy_state(c, out.h, out.hu, out.hv, c->buf(fin, 0), c->buf(fin, 1), c->buf(fin, 2), s)))
-    return rc;
+  if ((rc = stage_out(c, fin, out.h, out.hu, out.hv, s))) return rc;
   // coupled_forecast.py:152-156: a target not reached within 1e-6 s gives NaN.
   if (!(k.t >= t_target - 1.0e-6)) {
     const size_t N = c->cells();
@@ -1020,10 +1105,9 @@ int swe_loop_begin(SweContext* c, SweState state, double t0, int32_t max_steps, 
   c->loop_times[1] = HUGE_VAL;
   SWE_HIP_TRY(hipMemcpyAsync(c->d_times, c->loop_times, sizeof c->loop_times,
                              hipMemcpyHostToDevice, s));
-  if ((rc = copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), state.h, state.hu, state.hv, s)))
-    return rc;
+  if ((rc = stage_in(c, state, s))) return rc;
   c->loop_L = c->loop_const(2, max_steps);
-  launch_reduce(c, RED_INIT, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), c->loop_L, s);
+  launch_reduce(c, RED_INIT, state.h, state.hu, state.hv, c->loop_L, s);
   SWE_HIP_TRY(hipGetLastError());
   c->loop_active = true;
   return SWE_OK;
@@ -1043,6 +1127,7 @@ int swe_loop_end(SweContext* c, SweState state, SweRunInfo* info, void* stream) 
     return fail(SWE_EINVAL, "swe_loop_end: call swe_loop_begin first");
   hipStream_t s = (hipStream_t)stream;
   int rc;
+  c->loop_active = false;
   if ((rc = copy_final(c, state, s))) return rc;
   if (info) {
     SWE_HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopControl), hipMemcpyDeviceToHost, s));
