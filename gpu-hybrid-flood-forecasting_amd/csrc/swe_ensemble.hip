@@ -43,7 +43,7 @@ struct Member {
   Grid G;
   LoopConst L;
   LoopControl* ctl;
-  float* buf[2];  // two contiguous [h | hu | hv] work states
+  float* buf[2];  // two work states of 3N floats: [h | hu | hv], or packed when G.pk is set
   int segs_x, rows, n_waves, wave_base;
 };
 
@@ -73,7 +73,9 @@ __device__ __forceinline__ int find_member(const Member* __restrict__ mem, int n
   return lo;
 }
 
-template <class M>
+// PACKED (fast variant): members' work states and static fields in the packed
+// layout of the single-domain fast loop (swe_fast.hpp), same arithmetic.
+template <class M, bool PACKED = false>
 __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict__ mem,
                                                       int n_members, int total_waves,
                                                       EnsPartials P) {
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
     float* dst = mb.buf[par ^ 1];
     StripAcc acc;
     if constexpr (M::kFast)
-      march_strip_fast<true>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
+      march_strip_fast<true, PACKED>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
                              local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dt_f,
                              ctl->dtrate_f, ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr,
                              nullptr, nullptr, acc);
@@ -166,6 +168,7 @@ struct SweEnsemble {
   double* d_rain = nullptr;      // all members' series
   double* d_times = nullptr;     // per member {0, t_end}
   float* d_work = nullptr;       // all members' A|B states
+  float* d_pk = nullptr;         // fast variant: all members' packed static fields (3 per cell)
   EnsPartials P{};
   int parts_cap = 0;
   int total_waves = 0;
@@ -191,7 +194,7 @@ int layout(SweEnsemble* e, const std::vector<char>* running = nullptr) {
   if (e->variant == SWE_VARIANT_REFERENCE)
     ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<RefMath>, 256, 0));
   else
-    ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<FastMath>, 256, 0));
+    ENS_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ensemble_step<FastMath, true>, 256, 0));
   // Resident wavefronts: at most 4 workgroups (16 wavefronts) per CU under the LDS cap.
   const long slots = (long)cus * std::min(std::max(per_cu, 1), 4) * 4;
   auto waves_for = [&](int rows) {
@@ -255,8 +258,8 @@ void launch_ens(SweEnsemble* e, hipStream_t s) {
     hipLaunchKernelGGL(k_ensemble_step<RefMath>, dim3(e->grid), dim3(256), e->step_lds, s, e->d_mem,
                        (int)e->act.size(), e->total_waves, e->P);
   else
-    hipLaunchKernelGGL(k_ensemble_step<FastMath>, dim3(e->grid), dim3(256), e->step_lds, s, e->d_mem,
-                       (int)e->act.size(), e->total_waves, e->P);
+    hipLaunchKernelGGL((k_ensemble_step<FastMath, true>), dim3(e->grid), dim3(256), e->step_lds, s,
+                       e->d_mem, (int)e->act.size(), e->total_waves, e->P);
   hipLaunchKernelGGL(k_ensemble_finish, dim3((unsigned)e->act.size()), dim3(256), 0, s, e->d_mem,
                      e->P);
 }
@@ -286,6 +289,8 @@ int launch_chunks(SweEnsemble* e, int launches, hipStream_t s) {
 // state into the member's work buffer A and write per-workgroup partials of
 // the CFL maximum, the FP64 volume and the non-finite flag (slots
 // wave_base .. wave_base + gridDim.x - 1 of the partials, gridDim.x <= n_waves).
+// With G.pk set (fast variant) the state goes in packed and the member's
+// static fields are packed from its current arrays.
 template <class MM>
 __global__ __launch_bounds__(256) void k_ens_load(const Member* __restrict__ mem, int m,
                                                   const float* __restrict__ h,
@@ -301,10 +306,20 @@ __global__ __launch_bounds__(256) void k_ens_load(const Member* __restrict__ mem
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < N;
        k += (size_t)gridDim.x * blockDim.x) {
     const float a0 = h[k], a1 = hu[k], a2 = hv[k];
-    A[k] = a0;
-    A[N + k] = a1;
-    A[2 * N + k] = a2;
     const int a = G.mask[k] != 0;
+    if (G.pk) {
+      A[3 * k] = a0;
+      A[3 * k + 1] = a1;
+      A[3 * k + 2] = a2;
+      float* pk = const_cast<float*>(G.pk);
+      pk[3 * k] = G.bed[k];
+      pk[3 * k + 1] = a ? fabsf(G.n[k]) : -1.0f;
+      pk[3 * k + 2] = G.mult[k];
+    } else {
+      A[k] = a0;
+      A[N + k] = a1;
+      A[2 * N + k] = a2;
+    }
     inv = fmaxf(inv, cfl_inverse<MM>(G, a0, a1, a2, a));
     vol += a ? (double)a0 : 0.0;
     nf |= !(isfinite(a0) && isfinite(a1) && isfinite(a2));
@@ -362,14 +377,15 @@ __global__ __launch_bounds__(256) void k_ens_init(const Member* __restrict__ mem
   }
 }
 
-// Copy a member's final work state [h | hu | hv] out to the caller's arrays.
+// Copy a member's final work state ([h | hu | hv], or packed) out to the
+// caller's arrays.
 __global__ void k_ens_copy(const float* __restrict__ src, float* __restrict__ h,
-                           float* __restrict__ hu, float* __restrict__ hv, size_t N) {
+                           float* __restrict__ hu, float* __restrict__ hv, size_t N, int packed) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < N;
        k += (size_t)gridDim.x * blockDim.x) {
-    h[k] = src[k];
-    hu[k] = src[N + k];
-    hv[k] = src[2 * N + k];
+    h[k] = packed ? src[3 * k] : src[k];
+    hu[k] = packed ? src[3 * k + 1] : src[N + k];
+    hv[k] = packed ? src[3 * k + 2] : src[2 * N + k];
   }
 }
 
@@ -418,6 +434,7 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
       hipMalloc(&e->d_ctl, sizeof(LoopControl) * n) != hipSuccess ||
       hipHostMalloc(&e->h_ctl, sizeof(LoopControl) * n, hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&e->d_work, sizeof(float) * 6 * cells_total) != hipSuccess ||
+      (variant == SWE_VARIANT_FAST && hipMalloc(&e->d_pk, sizeof(float) * 3 * cells_total) != hipSuccess) ||
       hipMalloc(&e->d_mem, sizeof(Member) * n) != hipSuccess)
     return bail(efail(SWE_EHIP, "swe_ensemble_create: device allocation failed"));
   if (hipMemset(e->d_ctl, 0, sizeof(LoopControl) * n) != hipSuccess)
@@ -464,6 +481,7 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
     const size_t N = (size_t)p.nx * p.ny;
     mb.buf[0] = e->d_work + cell_off;
     mb.buf[1] = e->d_work + cell_off + 3 * N;
+    G.pk = e->d_pk ? e->d_pk + cell_off / 2 : nullptr;  // 3 floats per cell
     cell_off += 6 * N;
   }
   if (hipMemcpy(e->d_rain, rain_all.data(), sizeof(double) * rain_all.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -495,6 +513,7 @@ int swe_ensemble_destroy(SweEnsemble* e) {
   if (e->P.vol) (void)hipFree(e->P.vol);
   if (e->d_mem) (void)hipFree(e->d_mem);
   if (e->d_work) (void)hipFree(e->d_work);
+  if (e->d_pk) (void)hipFree(e->d_pk);
   if (e->h_ctl) (void)hipHostFree(e->h_ctl);
   if (e->d_ctl) (void)hipFree(e->d_ctl);
   if (e->d_times) (void)hipFree(e->d_times);
@@ -603,7 +622,7 @@ int swe_ensemble_end(SweEnsemble* e, SweRunInfo* infos, void* stream) {
     const size_t N = (size_t)e->host[m].G.nx * e->host[m].G.ny;
     const int nb = (int)std::min<size_t>((N + 255) / 256, 4096);
     hipLaunchKernelGGL(k_ens_copy, dim3(nb), dim3(256), 0, s, e->host[m].buf[k.steps & 1], st.h,
-                       st.hu, st.hv, N);
+                       st.hu, st.hv, N, e->d_pk ? 1 : 0);
     if (infos) {
       SweRunInfo& I = infos[m];
       I.outputs_written = k.out_idx;
