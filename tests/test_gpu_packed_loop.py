@@ -1,0 +1,118 @@
+"""The fast device loop's packed layout (state and static fields interleaved,
+the mask carried by the sign of |n|) against the caller-layout step.
+
+The loop packs the caller's arrays at every start and unpacks at the end; its
+arithmetic is the caller-layout step's, so a replay of the loop's recorded dt
+schedule with ``finite_volume_step`` must give the same bits -- also with
+negative Manning coefficients (the packed copy stores |n|; the friction law
+uses n^2) -- and fields changed in place between two runs must be used by the
+second run."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import random_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(case, dev):
+    from hybrid_flood_amd import numerics as nm
+
+    return nm.make_params(case["bed"], case["manning_n"], case["domain_mask"],
+                          case["rainfall_multiplier"], boundary_types=case["boundary_types"],
+                          rainfall_times_s=case["rain_t"], rainfall_rates_m_s=case["rain_r"],
+                          dx=case["dx"], dry_tolerance=case["dry"], device=dev)
+
+
+def _state(case, dev):
+    import torch
+
+    from hybrid_flood_amd import numerics as nm
+
+    return nm.SWEState(*(torch.as_tensor(case[k], device=dev).contiguous() for k in ("h", "hu", "hv")))
+
+
+@pytest.mark.parametrize("shape,sides", [((41, 77), (0, 1, 0, 0)), ((64, 130), (1, 1, 1, 1))])
+def test_loop_equals_step_replay_with_negative_n(cuda, shape, sides):
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd.adjoint import rollout_forward
+
+    case = random_case(17, shape, open_sides=sides, speed=0.8)
+    rng = np.random.default_rng(3)
+    flip = rng.uniform(size=shape) < 0.3
+    case["manning_n"] = np.where(flip, -case["manning_n"], case["manning_n"]).astype(np.float32)
+    params = _params(case, cuda)
+    times = np.array([0.0, 2.0, 5.0], np.float32)
+    _, res, tape = rollout_forward(params, _state(case, cuda), times, max_steps=3000, variant="fast",
+                                   record=False)
+    assert tape.steps > 10
+    cur = _state(case, cuda)
+    for k in range(tape.steps):
+        cur = nm.finite_volume_step(cur, params, tape.dts[k], tape.ts[k], variant="fast")
+    for got, want in zip((res.states.h[-1], res.states.hu[-1], res.states.hv[-1]), cur):
+        assert np.array_equal(got.cpu().numpy(), want.cpu().numpy())
+
+
+def test_fields_changed_in_place_are_used(cuda):
+    from hybrid_flood_amd import numerics as nm
+
+    case = random_case(23, (48, 70), open_sides=(0, 1, 0, 0), speed=0.8)
+    params = _params(case, cuda)
+    times = np.array([0.0, 3.0], np.float32)
+    first = nm.integrate_to_outputs(_state(case, cuda), params, times, max_steps=2000, variant="fast")
+    params.manning_n.mul_(1.7)
+    params.bed.add_(0.01)
+    params.rainfall_multiplier.mul_(0.5)
+    second = nm.integrate_to_outputs(_state(case, cuda), params, times, max_steps=2000, variant="fast")
+    case2 = dict(case, manning_n=params.manning_n.cpu().numpy(), bed=params.bed.cpu().numpy(),
+                 rainfall_multiplier=params.rainfall_multiplier.cpu().numpy())
+    fresh = nm.integrate_to_outputs(_state(case, cuda), _params(case2, cuda), times, max_steps=2000,
+                                    variant="fast")
+    for a, b in zip(second.states, fresh.states):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    assert not np.array_equal(first.states.h[-1].cpu().numpy(), second.states.h[-1].cpu().numpy())
+
+
+def test_variant_switch_on_one_context(cuda):
+    """One context runs fast (packed), reference (caller layout) and fast
+    loops in turn; each equals the run of a context of that variant alone, and
+    the variant cannot change while a loop is active."""
+    import torch
+
+    from hybrid_flood_amd import _lib
+    from hybrid_flood_amd import numerics as nm
+
+    case = random_case(29, (40, 66), open_sides=(1, 0, 0, 1), speed=0.8)
+    params = _params(case, cuda)
+    ctx = nm.context_for(params, "fast")
+    lib = _lib.load()
+
+    def run(variant):
+        ctx.set_variant(variant)
+        st = _state(case, cuda)
+        _lib.check(lib.swe_run_steps(ctx.handle, _lib.state_struct(*st), 25, 0.0, None,
+                                     _lib.stream_handle()), "swe_run_steps")
+        torch.cuda.synchronize()
+        return [x.cpu().numpy() for x in st]
+
+    fast1, ref, fast2 = run("fast"), run("reference"), run("fast")
+    alone = {}
+    for variant in ("fast", "reference"):
+        st = _state(case, cuda)
+        nm.run_steps(st, _params(case, cuda), 25, variant=variant)
+        alone[variant] = [x.cpu().numpy() for x in st]
+    for a, b, c in zip(fast1, fast2, alone["fast"]):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    for a, b in zip(ref, alone["reference"]):
+        assert np.array_equal(a, b)
+    st = _state(case, cuda)
+    _lib.check(lib.swe_loop_begin(ctx.handle, _lib.state_struct(*st), 0.0, 4, _lib.stream_handle()),
+               "swe_loop_begin")
+    with pytest.raises(ValueError):
+        ctx.set_variant("reference")
+    _lib.check(lib.swe_loop_end(ctx.handle, _lib.state_struct(*st), None, _lib.stream_handle()),
+               "swe_loop_end")
+    ctx.set_variant("fast")
