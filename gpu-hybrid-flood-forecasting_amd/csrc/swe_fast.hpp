@@ -401,11 +401,15 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
                                               so, 0);
       }
       if (LOOP) {
-        const size_t idx = (size_t)j * (size_t)G.nx + (size_t)ic;
-        for (int k = rec_lo; k < rec_hi; ++k) {
-          cube_h[(size_t)k * N + idx] = h_o;
-          cube_hu[(size_t)k * N + idx] = hu_o;
-          cube_hv[(size_t)k * N + idx] = hv_o;
+        // Output or checkpoint slots (rare steps; rec_lo/rec_hi are
+        // wave-uniform, so this is a scalar branch).
+        if (__builtin_expect(rec_lo < rec_hi, 0)) {
+          const size_t idx = (size_t)j * (size_t)G.nx + (size_t)ic;
+          for (int k = rec_lo; k < rec_hi; ++k) {
+            cube_h[(size_t)k * N + idx] = h_o;
+            cube_hu[(size_t)k * N + idx] = hu_o;
+            cube_hv[(size_t)k * N + idx] = hv_o;
+          }
         }
         // CFL of the new state (numerics.py:299-303): u = hu/max(h,dry) where
         // wet, and hu is already 0 where not wet.

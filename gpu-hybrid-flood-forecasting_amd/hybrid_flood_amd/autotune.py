@@ -30,6 +30,31 @@ from .numerics import SWEParams, SWEState, context_for
 CANDIDATE_ROWS = (4, 6, 8, 12, 14, 16, 17, 20, 21, 24, 32, 48, 64)
 # Resident wavefronts per compute unit: 0 = as many as registers allow.
 CANDIDATE_WAVES = (0, 20, 16, 12)
+# Resident wavefronts per compute unit of the fused loop step when registers
+# alone limit it (84 VGPRs: 5 per SIMD).
+UNCAPPED_WAVES = 20
+STRIP_COLS = 62
+
+
+def round_rows(shape, waves, cus: int, rounds=range(1, 9), lo: int = 4, hi: int = 64) -> list[int]:
+    """Strip heights whose wavefront count falls just under a whole number of
+    rounds of the resident capacity (cus x waves per CU), so that the last
+    round of a launch is (nearly) full: for k rounds, the strips per column
+    are floor(k * slots / segs_x) and the height is ceil(ny / strips).  For
+    many rounds the integer heights get coarse relative to a round, so the
+    fill is only approximate there; the tuner measures every candidate."""
+    ny, nx = int(shape[0]), int(shape[1])
+    segs = (nx + STRIP_COLS - 1) // STRIP_COLS
+    slots = cus * (waves if waves > 0 else UNCAPPED_WAVES)
+    out = set()
+    for k in rounds:
+        strips = (k * slots) // segs
+        if strips <= 0:
+            continue
+        r = -(-ny // strips)
+        if lo <= r <= hi:
+            out.add(r)
+    return sorted(out)
 
 
 def cache_path() -> Path:
@@ -82,8 +107,11 @@ def time_loop_rows(ctx, state: SWEState, rows: int, steps: int) -> float:
 
 
 def tune_rows(params: SWEParams, state: SWEState, *, variant: str | None = None, steps: int = 32,
-              candidates=CANDIDATE_ROWS, waves=CANDIDATE_WAVES, use_cache: bool = True) -> dict:
-    """Pick and apply the fastest (strip height, waves per CU) for this grid; returns the record."""
+              candidates=None, waves=CANDIDATE_WAVES, use_cache: bool = True) -> dict:
+    """Pick and apply the fastest (strip height, waves per CU) for this grid; returns the record.
+
+    ``candidates=None`` searches CANDIDATE_ROWS plus, for each cap, the
+    heights of round_rows; explicit candidates are searched as given."""
     from .numerics import DEFAULT_VARIANT
 
     variant = variant or DEFAULT_VARIANT
@@ -95,9 +123,12 @@ def tune_rows(params: SWEParams, state: SWEState, *, variant: str | None = None,
         chunk_steps = max(2, steps - steps % 2)
         ctx.set_graph_chunk(chunk_steps)
         timings = {}
+        cus = torch.cuda.get_device_properties(state.h.device).multi_processor_count
         for w in waves:
             ctx.set_waves_per_cu(int(w))
-            for r in candidates:
+            rows = (sorted(set(CANDIDATE_ROWS) | set(round_rows(state.h.shape, int(w), cus)))
+                    if candidates is None else [int(r) for r in candidates])
+            for r in rows:
                 timings[(int(r), int(w))] = time_loop_rows(ctx, state, int(r), chunk_steps)
         best = min(timings, key=timings.get)
         rec = {"rows": best[0], "waves_per_cu": best[1], "ms_per_step": timings[best],
