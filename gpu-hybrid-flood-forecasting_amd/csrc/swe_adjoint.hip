@@ -123,6 +123,11 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
       lu1 = ls1;
       lu2 = ls2;
       // ---- reverse of this lane's west x face (A = west cell, B = this cell) ----
+      // Only for the strip's own rows: on its two halo rows the x faces feed
+      // only that row's cotangent, which another strip owns and stores (the
+      // halo rows' lambda* above is what the boundary y faces need).  r is
+      // wave-uniform, so the branch is uniform and the DPP moves stay valid.
+      if (r >= j0 && r < j1) {
       const T lw0 = from_west(ls0), lw1 = from_west(ls1), lw2 = from_west(ls2);
       adj::SideGrad<T> dA, dB;
       adj::face_vjp<true, T, kFastAdj>(g, hg, dry, T(wh), T(whu), T(whv), T(wb), wa, T(Rcur.h),
@@ -147,6 +152,7 @@ __global__ __launch_bounds__(256) void k_adjoint_step(
       lu0 += from_east(sw0);
       lu1 += from_east(sw1);
       lu2 += from_east(sw2);
+      }
     }
     // ---- reverse of the y face r-1/2 (A = row r-1, B = row r) ----
     if (r >= j0) {
