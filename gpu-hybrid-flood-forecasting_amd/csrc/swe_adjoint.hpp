@@ -32,6 +32,16 @@
 #define SWE_HD
 #endif
 
+// The reverse pieces may fuse a*b+c into FMAs on the GPU: they carry no
+// bit-parity contract (the library is otherwise built with
+// -ffp-contract=off for the forward step's); the host build that finite
+// differences check stays uncontracted.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SWE_VJP_CONTRACT _Pragma("clang fp contract(fast)")
+#else
+#define SWE_VJP_CONTRACT
+#endif
+
 namespace swe {
 namespace adj {
 
@@ -123,6 +133,7 @@ template <bool XDIR, class T, bool FAST = false>
 SWE_HD inline void face_vjp(T g, T hg, T dry, T hA, T qnA, T qtA, T bA, int aA, T hB, T qnB,
                             T qtB, T bB, int aB, T gF0, T gFn, T gFt, T gcA, T gcB,
                             SideGrad<T>& dA, SideGrad<T>& dB) {
+  SWE_VJP_CONTRACT
   dA = {T(0), T(0), T(0)};
   dB = {T(0), T(0), T(0)};
   if (!(aA || aB)) return;  // irrelevant face: zero flux, no dependence
@@ -275,6 +286,7 @@ SWE_HD inline PointOut<T> point_fwd(T dry, T p, T hs, T hus, T hvs, T n, int act
 template <class T, bool FAST = false>
 SWE_HD inline PointVjp<T> point_vjp(T dry, T p, T hs, T hus, T hvs, T n, int active, T dtrate,
                                     T dtg, T lam_h, T lam_hu, T lam_hv) {
+  SWE_VJP_CONTRACT
   PointVjp<T> r{T(0), T(0), T(0), T(0), T(0)};
   if (!active) return r;
   const T hn = hs > T(0) ? hs : T(0);
