@@ -37,7 +37,7 @@ struct Grid {
   const uint8_t* __restrict__ mask;
   const float* __restrict__ mult;
   // Packed static fields of the fast device loop, 3 floats per cell: bed,
-  // |n| for an active cell or -1 for an inactive one, rain multiplier
+  // n^2 for an active cell or -1 for an inactive one, rain multiplier
   // (null where unused).
   const float* __restrict__ pk;
 };

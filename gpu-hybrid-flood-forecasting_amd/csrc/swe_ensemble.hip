@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_ens_load(const Member* __restrict__ mem
       A[3 * k + 2] = a2;
       float* pk = const_cast<float*>(G.pk);
       pk[3 * k] = G.bed[k];
-      pk[3 * k + 1] = a ? fabsf(G.n[k]) : -1.0f;
+      pk[3 * k + 1] = a ? G.n[k] * G.n[k] : -1.0f;
       pk[3 * k + 2] = G.mult[k];
     } else {
       A[k] = a0;

@@ -24,20 +24,42 @@
 
 namespace swe {
 
+// max(a, b) as one v_med3_f32 against +inf.  fmaxf costs one or two extra
+// v_max_f32 x, x, x on operands the compiler cannot prove canonical (loads,
+// cross-lane moves, selects, loop-carried values): it quiets signalling NaNs
+// first, which no value of this solver can be.  The +inf goes through an
+// empty asm statement (no instruction) so that the compiler cannot fold the
+// med3 back into a max.
+__device__ __forceinline__ float max_nc(float a, float b) {
+  float inf = __builtin_inff();
+  asm("" : "+s"(inf));
+  return __builtin_amdgcn_fmed3f(a, b, inf);
+}
+
+// True when p holds on every active lane of the wavefront.  The ballot of a
+// comparison is the comparison's own lane mask, so this costs scalar
+// instructions only.
+__device__ __forceinline__ bool wave_all(bool p) {
+  return __builtin_amdgcn_ballot_w64(p) == __builtin_amdgcn_read_exec();
+}
+
 struct SideF {
-  float h, eta, b, u, v, h2;
-  int a;
+  float h, eta, b, u, v;
+  float p;  // hydrostatic pressure term g/2 h^2, as (g/2 h) h
+  bool a;   // active (inside the domain mask); kept as a lane mask
 };
 
-__device__ __forceinline__ SideF make_side(float h, float hu, float hv, float b, int a,
-                                           float dry) {
+__device__ __forceinline__ SideF make_side(float h, float hu, float hv, float b, bool a,
+                                           float dry, float half_g) {
   SideF s;
-  const float r = __builtin_amdgcn_rcpf(fmaxf(h, dry));
+  // 1/h serves only where h > dry (elsewhere the velocity is selected to 0),
+  // so it needs no max(h, dry) guard.
+  const float r = __builtin_amdgcn_rcpf(h);
   const bool wet = h > dry;
   s.h = h;
   s.b = b;
   s.eta = h + b;
-  s.h2 = h * h;
+  s.p = (half_g * h) * h;
   s.u = wet ? hu * r : 0.0f;
   s.v = wet ? hv * r : 0.0f;
   s.a = a;
@@ -47,9 +69,9 @@ __device__ __forceinline__ SideF make_side(float h, float hu, float hv, float b,
 // un/ut of each side are (u, v) for x faces and (v, u) for y faces.
 template <bool XDIR, bool MASKED>
 __device__ __forceinline__ FaceFlux face_fast(const Grid& G, float hA, float etaA, float bA,
-                                              float unA, float utA, float h2A, int aA, float hB,
+                                              float unA, float utA, float pA, bool aA, float hB,
                                               float etaB, float bB, float unB, float utB,
-                                              float h2B, int aB) {
+                                              float pB, bool aB) {
   if (MASKED) {
     const bool wallA = aA && !aB;
     const bool wallB = !aA && aB;
@@ -58,16 +80,16 @@ __device__ __forceinline__ FaceFlux face_fast(const Grid& G, float hA, float eta
     bB = wallA ? bA : bB;
     unB = wallA ? -unA : unB;
     utB = wallA ? utA : utB;
-    h2B = wallA ? h2A : h2B;
+    pB = wallA ? pA : pB;
     hA = wallB ? hB : hA;
     etaA = wallB ? etaB : etaA;
     bA = wallB ? bB : bA;
     unA = wallB ? -unB : unA;
     utA = wallB ? utB : utA;
-    h2A = wallB ? h2B : h2A;
+    pA = wallB ? pB : pA;
   }
   const float dry = G.dry;
-  const float bf = fmaxf(bA, bB);
+  const float bf = max_nc(bA, bB);
   const float hrA = fmaxf(etaA - bf, 0.0f);
   const float hrB = fmaxf(etaB - bf, 0.0f);
   const bool wA = hrA > dry, wB = hrB > dry;
@@ -77,9 +99,11 @@ __device__ __forceinline__ FaceFlux face_fast(const Grid& G, float hA, float eta
   const float cB = __builtin_amdgcn_sqrtf(G.g * hrB);
   const float sL = fminf(fminf(vnA - cA, vnB - cB), 0.0f);
   const float sR = fmaxf(fmaxf(vnA + cA, vnB + cB), 0.0f);
-  const float hr2A = hrA * hrA, hr2B = hrB * hrB;
-  const float FnA = fmaf(QnA, vnA, G.half_g * hr2A);
-  const float FnB = fmaf(QnB, vnB, G.half_g * hr2B);
+  // g/2 hr^2 in the same rounding order as the cell's g/2 h^2, so that the
+  // source correction below is exactly zero where the reconstruction keeps h
+  const float prA = (G.half_g * hrA) * hrA, prB = (G.half_g * hrB) * hrB;
+  const float FnA = fmaf(QnA, vnA, prA);
+  const float FnB = fmaf(QnB, vnB, prB);
   float FtA, FtB, dQt;
   if (XDIR) {
     const float vtA = wA ? utA : 0.0f, vtB = wB ? utB : 0.0f;
@@ -101,8 +125,8 @@ __device__ __forceinline__ FaceFlux face_fast(const Grid& G, float hA, float eta
   F.f0 = fmaf(kc, hrB - hrA, ka * QnA + kb * QnB);
   F.fn = fmaf(kc, QnB - QnA, fmaf(ka, FnA, kb * FnB));
   F.ft = fmaf(kc, dQt, fmaf(ka, FtA, kb * FtB));
-  F.cA = G.half_g * (h2A - hr2A);
-  F.cB = G.half_g * (h2B - hr2B);
+  F.cA = pA - prA;
+  F.cB = pB - prB;
   if (MASKED) {
     const bool rel = aA || aB;
     F.f0 = rel ? F.f0 : 0.0f;
@@ -120,17 +144,17 @@ __device__ __forceinline__ FaceFlux face_fast_any(const Grid& G, const SideF& A,
   const float unA = XDIR ? A.u : A.v, utA = XDIR ? A.v : A.u;
   const float unB = XDIR ? B.u : B.v, utB = XDIR ? B.v : B.u;
   if (all_active)
-    return face_fast<XDIR, false>(G, A.h, A.eta, A.b, unA, utA, A.h2, A.a, B.h, B.eta, B.b, unB,
-                                  utB, B.h2, B.a);
-  return face_fast<XDIR, true>(G, A.h, A.eta, A.b, unA, utA, A.h2, A.a, B.h, B.eta, B.b, unB, utB,
-                               B.h2, B.a);
+    return face_fast<XDIR, false>(G, A.h, A.eta, A.b, unA, utA, A.p, A.a, B.h, B.eta, B.b, unB,
+                                  utB, B.p, B.a);
+  return face_fast<XDIR, true>(G, A.h, A.eta, A.b, unA, utA, A.p, A.a, B.h, B.eta, B.b, unB, utB,
+                               B.p, B.a);
 }
 
 // Raw inputs of one lane's cell in one row, as loaded (nothing is applied to
 // them until they are used, so the loads stay in flight).
 struct RawCell {
   float h, hu, hv, b;
-  unsigned a;
+  bool a;
   bool neg_hv;  // row beyond a reflective south/north edge
   float n, m;   // Manning n and rain multiplier (packed layout only)
 };
@@ -184,7 +208,7 @@ __device__ __forceinline__ StripIO strip_io(const Grid& G, const float* H, const
 
 // Packed layout of the fast device loop: 12 bytes per cell for the work
 // state (h, hu, hv) and 12 for the static fields (bed, n', multiplier), where
-// n' = |n| on an active cell and -1 on an inactive one -- the friction law
+// n' = n^2 on an active cell and -1 on an inactive one -- the friction law
 // uses n^2 only, and the sign carries the domain mask.  A row costs two
 // 96-bit loads per lane instead of seven scalar ones, and the step streams
 // three arrays through HBM instead of ten.
@@ -198,30 +222,54 @@ __device__ __forceinline__ StripIO strip_io(const Grid& G, const float* H, const
 struct PackedIO {
   const float* st;  // packed state
   const float* sx;  // packed static fields
+  float* out;       // packed output state
   unsigned off12;   // this lane's column, in bytes, in a packed row
   size_t row3;      // floats per packed row (3 nx)
 };
 
-__device__ __forceinline__ PackedIO packed_io(const Grid& G, const float* S, int ic) {
+// A wave-uniform pointer, read from lane 0 so that it sits in scalar
+// registers even where the compiler holds it in vector ones (a pointer
+// loaded from device memory that the kernel also writes).
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (T*)(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ PackedIO packed_io(const Grid& G, const float* S, float* O, int ic) {
   PackedIO io;
-  io.st = S;
-  io.sx = G.pk;
+  io.st = uniform_ptr(S);
+  io.sx = uniform_ptr(G.pk);
+  io.out = uniform_ptr(O);
   io.off12 = (unsigned)ic * 12u;
   io.row3 = 3 * (size_t)G.nx;
   return io;
 }
 
-__device__ __forceinline__ const float* packed_at(const float* base, size_t row3, int jj,
-                                                  unsigned off12) {
-  return (const float*)((const char*)(base + (size_t)jj * row3) + off12);
+// base, row3 and jj must be wave-uniform.  The row's address is pinned in
+// scalar registers (an empty asm statement, no instruction) so that the
+// compiler keeps the scalar-base + 32-bit lane-offset form of the global
+// load/store instead of folding the loop-invariant lane offset into a 64-bit
+// per-lane pointer that costs a vector add per row.
+using GlobalF = __attribute__((address_space(1))) float;
+
+__device__ __forceinline__ const GlobalF* packed_at(const float* base, size_t row3, int jj,
+                                                    unsigned off12) {
+  const __attribute__((address_space(1))) char* row =
+      (const __attribute__((address_space(1))) char*)(base + (size_t)jj * row3);
+  asm("" : "+s"(row));
+  asm("" : "+v"(off12));  // keeps the 32-bit offset visible at the access
+  return (const GlobalF*)(row + off12);
 }
 
 // Row j of the lane's column: clamped row index (numerics.py:29-55); the
 // reflective-edge sign flips are applied in side_of.
 __device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, int j) {
   const int jj = min(max(j, 0), G.ny - 1);
-  const float* st = packed_at(io.st, io.row3, jj, io.off12);
-  const float* sx = packed_at(io.sx, io.row3, jj, io.off12);
+  const GlobalF* st = packed_at(io.st, io.row3, jj, io.off12);
+  const GlobalF* sx = packed_at(io.sx, io.row3, jj, io.off12);
   RawCell r;
   r.h = st[0];
   r.hu = st[1];
@@ -229,7 +277,7 @@ __device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, i
   r.b = sx[0];
   r.n = sx[1];
   r.m = sx[2];
-  r.a = signbit(r.n) ? 0u : 1u;  // sign bit clear: active
+  r.a = !signbit(r.n);  // sign bit clear: active
   r.neg_hv = (j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0);
   return r;
 }
@@ -241,7 +289,7 @@ __device__ __forceinline__ RawCell load_raw(const Grid& G, const StripIO& io, in
   r.hu = io.ld(io.hu, jj);
   r.hv = io.ld(io.hv, jj);
   r.b = io.ld(io.bed, jj);
-  r.a = io.ld_mask(jj);
+  r.a = io.ld_mask(jj) != 0;
   r.neg_hv = (j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0);
   return r;
 }
@@ -253,11 +301,12 @@ struct RowCell {
 
 // Cell quantities of a loaded row, with the normal momentum negated beyond a
 // reflective edge (numerics.py:29-55).
-__device__ __forceinline__ RowCell side_of(const RawCell& r, bool neg_hu, float dry) {
+__device__ __forceinline__ RowCell side_of(const RawCell& r, bool neg_hu, float dry,
+                                           float half_g) {
   RowCell c;
   const float hu = neg_hu ? -r.hu : r.hu;
   const float hv = r.neg_hv ? -r.hv : r.hv;
-  c.s = make_side(r.h, hu, hv, r.b, r.a != 0, dry);
+  c.s = make_side(r.h, hu, hv, r.b, r.a, dry, half_g);
   c.hu = hu;
   c.hv = hv;
   return c;
@@ -293,7 +342,7 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
   using IO = std::conditional_t<PACKED, PackedIO, StripIO>;
   IO io;
   if constexpr (PACKED)
-    io = packed_io(G, H, ic);
+    io = packed_io(G, H, Ho, ic);
   else
     io = strip_io(G, H, HU, HV, max(j0 - 1, 0), ic);
   const size_t out0 = (size_t)j0 * (size_t)G.nx, out_cells = (size_t)(j1 - j0) * (size_t)G.nx;
@@ -305,18 +354,18 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
   }
 
   // Prologue: the south face of row j0, rows j0 (current) and j0+1 (in flight).
-  RowCell cur = side_of(load_raw(G, io, j0 - 1), neg_hu, dry);
+  RowCell cur = side_of(load_raw(G, io, j0 - 1), neg_hu, dry, G.half_g);
   RawCell next = load_raw(G, io, j0);
   float yt0, yt1, yt2;
   float nn, mult;
   {
-    const RowCell c0 = side_of(next, neg_hu, dry);
+    const RowCell c0 = side_of(next, neg_hu, dry, G.half_g);
     if constexpr (PACKED) {
       nn = next.n;
       mult = next.m;
     }
     next = load_raw(G, io, j0 + 1);
-    const FaceFlux fy = face_fast_any<false>(G, cur.s, c0.s, __all(cur.s.a && c0.s.a));
+    const FaceFlux fy = face_fast_any<false>(G, cur.s, c0.s, wave_all(cur.s.a && c0.s.a));
     if (LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
     yt0 = fy.f0;
     yt1 = fy.ft;
@@ -346,15 +395,32 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
       mult_next = io.ld(io.mult, min(j + 1, j1 - 1));
     }
 
-    SideF w;
-    w.h = from_west(cur.s.h);
-    w.eta = from_west(cur.s.eta);
-    w.b = from_west(cur.s.b);
-    w.u = from_west(cur.s.u);
-    w.v = from_west(cur.s.v);
-    w.h2 = from_west(cur.s.h2);
-    w.a = from_west(cur.s.a);
-    const FaceFlux fx = face_fast_any<true>(G, w, cur.s, __all(lane == 0 || (w.a && cur.s.a)));
+    // Active flag of row j.  In the packed layout it is re-derived from the
+    // sign of row j's n' (one compare, straight into a lane mask) rather than
+    // carried over from the previous row, which the compiler does through a
+    // vector register and two conversions.
+    // The empty asm keeps the compiler from folding the compare into the
+    // loop's phi nodes, which would move it to the loop edge and make the
+    // edge wait for the loads in flight.
+    if constexpr (PACKED) asm("" : "+v"(nn_j));
+    const bool ca = PACKED ? !signbit(nn_j) : cur.s.a;
+
+    // x face between lane-1 (A) and this lane (B).  Every face of lanes
+    // 1..63 joins two active cells when the active mask, shifted by one
+    // lane, still covers those lanes: scalar work on the ballot.
+    const float weta = from_west(cur.s.eta), wb = from_west(cur.s.b), wu = from_west(cur.s.u),
+                wv = from_west(cur.s.v), wp = from_west(cur.s.p);
+    const unsigned long long am = __builtin_amdgcn_ballot_w64(ca);
+    const unsigned long long need = __builtin_amdgcn_read_exec() & ~1ull;
+    FaceFlux fx;
+    if (((am << 1) & am & need) == need) {
+      fx = face_fast<true, false>(G, 0.0f, weta, wb, wu, wv, wp, true, cur.s.h, cur.s.eta,
+                                  cur.s.b, cur.s.u, cur.s.v, cur.s.p, true);
+    } else {
+      const bool wa = from_west((int)ca) != 0;
+      fx = face_fast<true, true>(G, from_west(cur.s.h), weta, wb, wu, wv, wp, wa, cur.s.h,
+                                 cur.s.eta, cur.s.b, cur.s.u, cur.s.v, cur.s.p, ca);
+    }
     const float xr0 = fx.f0, xr1 = fx.fn + fx.cB, xr2 = fx.ft;
     const float xl0 = from_east(fx.f0);
     const float xl1 = from_east(fx.fn + fx.cA);
@@ -363,8 +429,14 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
       if (i == 0) acc.bnd += (double)fx.f0 * dyd;
       if (i == G.nx) acc.bnd -= (double)fx.f0 * dyd;
     }
-    up = side_of(north, neg_hu, dry);
-    const FaceFlux gy = face_fast_any<false>(G, cur.s, up.s, __all(cur.s.a && up.s.a));
+    up = side_of(north, neg_hu, dry, G.half_g);
+    FaceFlux gy;
+    if (wave_all(ca && up.s.a))
+      gy = face_fast<false, false>(G, cur.s.h, cur.s.eta, cur.s.b, cur.s.v, cur.s.u, cur.s.p, true,
+                                   up.s.h, up.s.eta, up.s.b, up.s.v, up.s.u, up.s.p, true);
+    else
+      gy = face_fast<false, true>(G, cur.s.h, cur.s.eta, cur.s.b, cur.s.v, cur.s.u, cur.s.p, ca,
+                                  up.s.h, up.s.eta, up.s.b, up.s.v, up.s.u, up.s.p, up.s.a);
     const float yb0 = gy.f0, yb1 = gy.ft, yb2 = gy.fn + gy.cA;
 
     const float d0 = fmaf(yb0 - yt0, rdy, (xl0 - xr0) * rdx);
@@ -374,20 +446,21 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
     const float hus = fmaf(-dt, d1, cur.hu);
     const float hvs = fmaf(-dt, d2, cur.hv);
     const float hn = fmaxf(hs, 0.0f);
-    const float hd = fmaxf(hn, dry);
+    const float hd = fmaxf(hs, dry);  // = max(hn, dry): dry > 0
     const float rhd = __builtin_amdgcn_rcpf(hd);
     const float q = __builtin_amdgcn_sqrtf(fmaf(hus, hus, hvs * hvs));
     // K = dtg n^2 q / hd^(7/3), the power as 2^(-7/3 log2 hd) on the hardware log/exp
     const float ip73 = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(hd) * -kPow73);
-    const float K = ((dtg * (nn_j * nn_j)) * q) * ip73;
+    const float n2 = PACKED ? nn_j : nn_j * nn_j;  // the packed field holds n^2
+    const float K = ((dtg * n2) * q) * ip73;
     const float rD = __builtin_amdgcn_rcpf(1.0f + K);
-    const bool wet = cur.s.a && (hn > dry);
-    const float h_o = cur.s.a ? hn : 0.0f;
+    const bool wet = ca && (hn > dry);
+    const float h_o = ca ? hn : 0.0f;
     const float hu_o = wet ? hus * rD : 0.0f;
     const float hv_o = wet ? hvs * rD : 0.0f;
     if (own) {
       if constexpr (PACKED) {
-        float* o = const_cast<float*>(packed_at(Ho, io.row3, j, io.off12));
+        GlobalF* o = const_cast<GlobalF*>(packed_at(io.out, io.row3, j, io.off12));
         o[0] = h_o;
         o[1] = hu_o;
         o[2] = hv_o;
@@ -404,7 +477,12 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
         // Output or checkpoint slots (rare steps; rec_lo/rec_hi are
         // wave-uniform, so this is a scalar branch).
         if (__builtin_expect(rec_lo < rec_hi, 0)) {
-          const size_t idx = (size_t)j * (size_t)G.nx + (size_t)ic;
+          // j through an empty asm statement: keeps the compiler from
+          // strength-reducing these addresses into per-lane pointers that
+          // every row would advance.
+          int jr = j;
+          asm("" : "+s"(jr));
+          const size_t idx = (size_t)jr * (size_t)G.nx + (size_t)ic;
           for (int k = rec_lo; k < rec_hi; ++k) {
             cube_h[(size_t)k * N + idx] = h_o;
             cube_hu[(size_t)k * N + idx] = hu_o;
@@ -415,9 +493,9 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
         // wet, and hu is already 0 where not wet.
         const float ws = __builtin_amdgcn_sqrtf(G.g * h_o);
         const float inv = fmaf(fabsf(hv_o * rhd) + ws, rdy, (fabsf(hu_o * rhd) + ws) * rdx);
-        acc.inv = fmaxf(acc.inv, cur.s.a ? inv : 0.0f);
+        acc.inv = max_nc(acc.inv, ca ? inv : 0.0f);
         acc.vol += (double)h_o;
-        clip32 += cur.s.a ? fmaxf(-hs, 0.0f) : 0.0f;
+        clip32 += ca ? hn - hs : 0.0f;  // max(-hs, 0), exactly
         // (x + y + z) * 0 stays 0 unless one of them is infinite or NaN
         nf_probe = fmaf(h_o + hu_o + hv_o, 0.0f, nf_probe);
       }
@@ -427,17 +505,22 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
     yt2 = gy.fn + gy.cB;
   };
 
-  // Two rows per trip with the carried values alternating between two sets
-  // of registers, so nothing is copied at the loop edge.
-  RowCell alt;
-  RawCell next2;
-  float nn2 = 0.0f, mult2 = 0.0f;
+  // Three rows per trip with the carried values rotating through three sets
+  // of registers.  A row's raw inputs stay live over three rows (loaded two
+  // rows ahead, read as the north neighbour, then as the current row), so
+  // with two sets the compiler would copy one set at every loop edge, and
+  // wait there for the loads in flight.
+  RowCell c1, c2;
+  RawCell r1, r2;
+  float nn1 = 0.0f, mult1 = 0.0f, nn2 = 0.0f, mult2 = 0.0f;
   int j = j0;
-  for (; j + 1 < j1; j += 2) {
-    row(j, cur, alt, next, next2, nn, mult, nn2, mult2);
-    row(j + 1, alt, cur, next2, next, nn2, mult2, nn, mult);
+  for (; j + 2 < j1; j += 3) {
+    row(j, cur, c1, next, r1, nn, mult, nn1, mult1);
+    row(j + 1, c1, c2, r1, r2, nn1, mult1, nn2, mult2);
+    row(j + 2, c2, cur, r2, next, nn2, mult2, nn, mult);
   }
-  if (j < j1) row(j, cur, alt, next, next2, nn, mult, nn2, mult2);
+  if (j < j1) row(j, cur, c1, next, r1, nn, mult, nn1, mult1);
+  if (j + 1 < j1) row(j + 1, c1, c2, r1, r2, nn1, mult1, nn2, mult2);
   // yt0 now holds the mass flux through the north face of row j1-1.
   if (LOOP && j1 == G.ny && own) acc.bnd -= (double)yt0 * dxd;
   if (LOOP) {

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
     Grid G, const float* __restrict__ H, const float* __restrict__ HU,
     const float* __restrict__ HV, float* __restrict__ Ho, float* __restrict__ HUo,
     float* __restrict__ HVo, float dt_arg, float dtrate_arg, float dtg_arg, int rows,
-    int segs_x, int n_waves, LoopControl* __restrict__ ctl, LoopConst L, Partials P,
+    int segs_x, int n_waves, int rev, LoopControl* __restrict__ ctl, LoopConst L, Partials P,
     float* __restrict__ cube_h, float* __restrict__ cube_hu, float* __restrict__ cube_hv) {
   float dt = dt_arg, dtrate = dtrate_arg, dtg = dtg_arg;
   int rec_lo = 0, rec_hi = 0;
@@ -65,7 +65,12 @@ __global__ __launch_bounds__(kBlock) void k_step(
     }
   }
   static_assert(!PACKED || (M::kFast && MODE != MODE_STEP), "packed layout: fast device loop only");
-  const int gw = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave_in_block();
+  int gw = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave_in_block();
+  // rev: this step sweeps the strips in the opposite order, so that its
+  // first strips are the ones the previous step touched last and still has
+  // in the memory-side cache.  Strip order does not change any cell's
+  // arithmetic.
+  if (rev && gw < n_waves) gw = n_waves - 1 - gw;
   StripAcc acc;
   if (gw < n_waves) {
     if constexpr (M::kFast)
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_static(Grid G, float* __restric
   for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
        k += (size_t)gridDim.x * kBlock) {
     pk[3 * k] = G.bed[k];
-    pk[3 * k + 1] = G.mask[k] ? fabsf(G.n[k]) : -1.0f;
+    pk[3 * k + 1] = G.mask[k] ? G.n[k] * G.n[k] : -1.0f;
     pk[3 * k + 2] = G.mult[k];
   }
 }
@@ -305,6 +310,8 @@ struct SweContext {
   hipGraphExec_t graph = nullptr;
   std::vector<const void*> graph_key;
   int split_finish = 1;      // finish each loop step in its own one-workgroup kernel
+  int alt_sweep = 1;         // odd device-loop steps sweep the strips in reverse order
+  int sweep_rev = 0;         // reverse flag of the step launch being issued
   double* dt_log = nullptr;  // caller's device buffer for the dt schedule
   int dt_log_cap = 0;
   float *ck_h = nullptr, *ck_hu = nullptr, *ck_hv = nullptr;  // caller's checkpoint cube
@@ -436,7 +443,7 @@ void launch_step(SweContext* c, const float* h, const float* hu, const float* hv
                  float* ch, float* chu, float* chv, hipStream_t s) {
   hipLaunchKernelGGL((k_step<M, MODE, PACKED>), dim3(c->step_blocks()), dim3(kBlock), c->step_lds, s, c->G, h, hu,
                      hv, ho, huo, hvo, dt, dtrate, dtg, c->rows, c->segs_x(), c->n_waves(),
-                     c->d_ctl, L, c->P, ch, chu, chv);
+                     c->sweep_rev, c->d_ctl, L, c->P, ch, chu, chv);
 }
 
 void launch_step_any(SweContext* c, int mode, const float* h, const float* hu, const float* hv,
@@ -512,6 +519,11 @@ double host_interp(const SweContext* c, double t) {
 // of the current loop.
 void launch_loop_step(SweContext* c, int a, const LoopConst& L, float* ch, float* chu, float* chv,
                       hipStream_t s) {
+  c->sweep_rev = c->alt_sweep ? a : 0;
+  struct Reset {
+    SweContext* c;
+    ~Reset() { c->sweep_rev = 0; }
+  } reset{c};
   if (c->loop_packed)
     launch_step_any(c, MODE_LOOP, c->ws(a), nullptr, nullptr, c->ws(1 - a), nullptr, nullptr, 0.f,
                     0.f, 0.f, L, ch, chu, chv, s);
@@ -530,6 +542,7 @@ int launch_loop_chunk(SweContext* c, int count, const LoopConst& L, float* ch, f
                                     (const void*)(intptr_t)L.n_times,
                                     (const void*)(intptr_t)L.max_steps,
                                     (const void*)(intptr_t)c->split_finish,
+                                    (const void*)(intptr_t)c->alt_sweep,
                                     (const void*)(intptr_t)c->step_lds,
                                     (const void*)(intptr_t)c->loop_packed};
     // The loop constants are baked into the captured kernel arguments.
@@ -869,6 +882,13 @@ int swe_set_loop_finish(SweContext* c, int32_t split) {
     c->graph = nullptr;
     c->graph_key.clear();
   }
+  return SWE_OK;
+}
+
+int swe_set_sweep_order(SweContext* c, int32_t alternate) {
+  if (!c || (alternate != 0 && alternate != 1))
+    return fail(SWE_EINVAL, "swe_set_sweep_order: alternate must be 0 or 1");
+  c->alt_sweep = alternate;
   return SWE_OK;
 }
 

@@ -99,6 +99,7 @@ SIGNATURES = {
     "swe_set_graph_chunk": (c_int32, [c_void_p, c_int32]),
     "swe_set_dt_log": (c_int32, [c_void_p, c_void_p, c_int32]),
     "swe_set_loop_finish": (c_int32, [c_void_p, c_int32]),
+    "swe_set_sweep_order": (c_int32, [c_void_p, c_int32]),
     "swe_adjoint_step": (c_int32, [c_void_p, SweState, SweState, SweState, c_double, c_double,
                                    c_void_p, c_void_p, c_void_p]),
     "swe_adjoint_sweep": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
@@ -292,6 +293,9 @@ class Context:
 
     def set_loop_finish(self, split: int) -> None:
         check(load().swe_set_loop_finish(self.handle, int(split)), "swe_set_loop_finish")
+
+    def set_sweep_order(self, alternate: int) -> None:
+        check(load().swe_set_sweep_order(self.handle, int(alternate)), "swe_set_sweep_order")
 
     def set_graph_chunk(self, chunk: int) -> None:
         check(load().swe_set_graph_chunk(self.handle, int(chunk)), "swe_set_graph_chunk")

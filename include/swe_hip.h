@@ -130,6 +130,12 @@ int swe_set_adjoint_rows(SweContext* ctx, int32_t rows);
  * them and plans the next step; 0 = the last workgroup of the step kernel does
  * it (one launch per step, but a release/acquire across workgroups). */
 int swe_set_loop_finish(SweContext* ctx, int32_t split);
+/* Strip order of the device loop (tuning knob): 1 (default) = odd steps
+ * sweep the strips in reverse, so each step starts where the previous one
+ * ended and reads what is still in the memory-side cache; 0 = every step
+ * sweeps in the same order.  States are bit-identical either way.  No
+ * reference counterpart (XLA schedules the reference's step). */
+int swe_set_sweep_order(SweContext* ctx, int32_t alternate);
 
 /* One step with caller-given dt and time (finite_volume_step,
  * jax_solver/numerics.py:312-369).  `in` and `out` must not alias. */

@@ -1,12 +1,14 @@
 """The fast device loop's packed layout (state and static fields interleaved,
-the mask carried by the sign of |n|) against the caller-layout step.
+the mask carried by the sign of the packed n^2 field) against the
+caller-layout step.
 
 The loop packs the caller's arrays at every start and unpacks at the end; its
 arithmetic is the caller-layout step's, so a replay of the loop's recorded dt
 schedule with ``finite_volume_step`` must give the same bits -- also with
-negative Manning coefficients (the packed copy stores |n|; the friction law
-uses n^2) -- and fields changed in place between two runs must be used by the
-second run."""
+negative Manning coefficients (the packed copy stores n^2, as the friction law
+uses it) -- and fields changed in place between two runs must be used by the
+second run.  The loop's alternating strip order (odd steps sweep in reverse)
+must not change a bit either."""
 
 from __future__ import annotations
 
@@ -54,6 +56,32 @@ def test_loop_equals_step_replay_with_negative_n(cuda, shape, sides):
         cur = nm.finite_volume_step(cur, params, tape.dts[k], tape.ts[k], variant="fast")
     for got, want in zip((res.states.h[-1], res.states.hu[-1], res.states.hv[-1]), cur):
         assert np.array_equal(got.cpu().numpy(), want.cpu().numpy())
+
+
+@pytest.mark.parametrize("rows", [5, 16])
+def test_sweep_order_does_not_change_the_state(cuda, rows):
+    from hybrid_flood_amd import numerics as nm
+
+    case = random_case(29, (90, 150), open_sides=(1, 0, 1, 0), speed=0.8)
+    params = _params(case, cuda)
+    times = np.array([0.0, 1.0, 4.0], np.float32)
+    ctx = nm.context_for(params, "fast")
+    ctx.set_tile_rows(rows)
+    try:
+        runs = []
+        for alternate in (1, 0, 1):
+            ctx.set_sweep_order(alternate)
+            runs.append(nm.integrate_to_outputs(_state(case, cuda), params, times, max_steps=3000,
+                                                variant="fast"))
+    finally:
+        ctx.set_sweep_order(1)
+        ctx.set_tile_rows(0)
+    for other in runs[1:]:
+        assert other.outputs_written == runs[0].outputs_written == len(times)
+        for a, b in zip(runs[0].states, other.states):
+            assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    with pytest.raises(ValueError):
+        ctx.set_sweep_order(2)
 
 
 def test_fields_changed_in_place_are_used(cuda):
