@@ -2,7 +2,7 @@
 
 from __future__ import annotations
 
-from hybrid_flood_amd.autotune import round_rows
+from hybrid_flood_amd.autotune import UNCAPPED_WAVES, round_rows
 
 
 def test_round_rows_fill_whole_rounds():
@@ -21,6 +21,6 @@ def test_round_rows_fill_whole_rounds():
 
 
 def test_round_rows_bounds_and_uncapped():
-    assert round_rows((4096, 4096), 0, 256) == round_rows((4096, 4096), 20, 256)
+    assert round_rows((4096, 4096), 0, 256) == round_rows((4096, 4096), UNCAPPED_WAVES, 256)
     assert all(4 <= r <= 64 for r in round_rows((300, 5000), 12, 256))
     assert round_rows((8, 8), 16, 256) == []
