@@ -317,8 +317,8 @@ __device__ __forceinline__ RowCell side_of(const RawCell& r, bool neg_hu, float 
 // row j+2 are in flight while row j is computed.  PACKED: H and Ho are packed
 // states (3 floats per cell, HU/HV unused) and the static fields come from
 // G.pk; the arithmetic is the same, so both layouts give identical bits.
-template <bool LOOP, bool PACKED = false>
-__device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __restrict__ H,
+template <bool LOOP, bool PACKED, bool SOUTH>
+__device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
                                                  const float* __restrict__ HV,
                                                  float* __restrict__ Ho, float* __restrict__ HUo,
@@ -353,10 +353,14 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
     ohv = make_rsrc(HVo + out0, out_cells * 4);
   }
 
-  // Prologue: the south face of row j0, rows j0 (current) and j0+1 (in flight).
-  RowCell cur = side_of(load_raw(G, io, j0 - 1), neg_hu, dry, G.half_g);
-  RawCell next = load_raw(G, io, j0);
-  float yt0, yt1, yt2;
+  // The march runs north (rows j0 .. j1-1) or, SOUTH, the other way
+  // (j1-1 .. j0); dj is the step.  "Ahead" below means the march direction.
+  constexpr int dj = SOUTH ? -1 : 1;
+  const int jf = SOUTH ? j1 - 1 : j0;  // first row of the march
+  // Prologue: the face behind row jf, rows jf (current) and jf+dj (in flight).
+  RowCell cur = side_of(load_raw(G, io, jf - dj), neg_hu, dry, G.half_g);
+  RawCell next = load_raw(G, io, jf);
+  float yc0, yc1, yc2;  // carried y face: between the current row and the one behind it
   float nn, mult;
   {
     const RowCell c0 = side_of(next, neg_hu, dry, G.half_g);
@@ -364,35 +368,45 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
       nn = next.n;
       mult = next.m;
     }
-    next = load_raw(G, io, j0 + 1);
-    const FaceFlux fy = face_fast_any<false>(G, cur.s, c0.s, wave_all(cur.s.a && c0.s.a));
-    if (LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
-    yt0 = fy.f0;
-    yt1 = fy.ft;
-    yt2 = fy.fn + fy.cB;
+    next = load_raw(G, io, jf + dj);
+    if constexpr (!SOUTH) {  // south face of row j0
+      const FaceFlux fy = face_fast_any<false>(G, cur.s, c0.s, wave_all(cur.s.a && c0.s.a));
+      if (LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
+      yc0 = fy.f0;
+      yc1 = fy.ft;
+      yc2 = fy.fn + fy.cB;
+    } else {  // north face of row j1-1
+      const FaceFlux fy = face_fast_any<false>(G, c0.s, cur.s, wave_all(cur.s.a && c0.s.a));
+      if (LOOP && j1 == G.ny && own) acc.bnd -= (double)fy.f0 * dxd;
+      yc0 = fy.f0;
+      yc1 = fy.ft;
+      yc2 = fy.fn + fy.cA;
+    }
     cur = c0;
   }
   if constexpr (!PACKED) {
-    nn = io.ld(io.n, j0);
-    mult = io.ld(io.mult, j0);
+    nn = io.ld(io.n, jf);
+    mult = io.ld(io.mult, jf);
   }
   float nf_probe = 0.0f;
   float clip32 = 0.0f;  // clip depth is small; float32 per lane over the strip
 
-  // Row j: `cur` is row j, `north` row j+1 (loaded one row ago).  Issues the
-  // loads of row j+2 into `ahead` and of row j+1's n/multiplier into
-  // nn_next/mult_next, and leaves row j+1's side in `up`.
+  // Row j: `cur` is row j, `north` row j+dj (the next row of the march,
+  // loaded one row ago).  Issues the loads of row j+2dj into `ahead` and of
+  // row j+dj's n/multiplier into nn_next/mult_next, and leaves row j+dj's
+  // side in `up`.
   auto row = [&](int j, const RowCell& cur, RowCell& up, const RawCell& north, RawCell& ahead,
                  float nn_j, float mult_j, float& nn_next, float& mult_next) {
     // Unconditional, so the wait counts stay exact; past the strip the
     // clamped rows were just loaded and come from the cache.
-    ahead = load_raw(G, io, min(j + 2, j1));
-    if constexpr (PACKED) {  // row j+1's fields arrived with its state
+    ahead = load_raw(G, io, SOUTH ? max(j - 2, j0 - 1) : min(j + 2, j1));
+    if constexpr (PACKED) {  // row j+dj's fields arrived with its state
       nn_next = north.n;
       mult_next = north.m;
     } else {
-      nn_next = io.ld(io.n, min(j + 1, j1 - 1));
-      mult_next = io.ld(io.mult, min(j + 1, j1 - 1));
+      const int jn = SOUTH ? max(j - 1, j0) : min(j + 1, j1 - 1);
+      nn_next = io.ld(io.n, jn);
+      mult_next = io.ld(io.mult, jn);
     }
 
     // Active flag of row j.  In the packed layout it is re-derived from the
@@ -429,19 +443,27 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
       if (i == 0) acc.bnd += (double)fx.f0 * dyd;
       if (i == G.nx) acc.bnd -= (double)fx.f0 * dyd;
     }
+    // The y face ahead of row j; a y face's A side is always its south cell.
     up = side_of(north, neg_hu, dry, G.half_g);
+    const SideF& ys = SOUTH ? up.s : cur.s;  // south cell of the face
+    const SideF& yn = SOUTH ? cur.s : up.s;  // north cell
+    const bool as = SOUTH ? up.s.a : ca, an = SOUTH ? ca : up.s.a;
     FaceFlux gy;
     if (wave_all(ca && up.s.a))
-      gy = face_fast<false, false>(G, cur.s.h, cur.s.eta, cur.s.b, cur.s.v, cur.s.u, cur.s.p, true,
-                                   up.s.h, up.s.eta, up.s.b, up.s.v, up.s.u, up.s.p, true);
+      gy = face_fast<false, false>(G, ys.h, ys.eta, ys.b, ys.v, ys.u, ys.p, true, yn.h, yn.eta,
+                                   yn.b, yn.v, yn.u, yn.p, true);
     else
-      gy = face_fast<false, true>(G, cur.s.h, cur.s.eta, cur.s.b, cur.s.v, cur.s.u, cur.s.p, ca,
-                                  up.s.h, up.s.eta, up.s.b, up.s.v, up.s.u, up.s.p, up.s.a);
-    const float yb0 = gy.f0, yb1 = gy.ft, yb2 = gy.fn + gy.cA;
+      gy = face_fast<false, true>(G, ys.h, ys.eta, ys.b, ys.v, ys.u, ys.p, as, yn.h, yn.eta, yn.b,
+                                  yn.v, yn.u, yn.p, an);
+    // Row j's share of the new face (its own source correction) and the
+    // next row's share, carried to it.
+    const float yf0 = gy.f0, yf1 = gy.ft, yf2 = gy.fn + (SOUTH ? gy.cB : gy.cA);
+    const float yn0 = SOUTH ? yc0 : yf0, yn1 = SOUTH ? yc1 : yf1, yn2 = SOUTH ? yc2 : yf2;  // north
+    const float ys0 = SOUTH ? yf0 : yc0, ys1 = SOUTH ? yf1 : yc1, ys2 = SOUTH ? yf2 : yc2;  // south
 
-    const float d0 = fmaf(yb0 - yt0, rdy, (xl0 - xr0) * rdx);
-    const float d1 = fmaf(yb1 - yt1, rdy, (xl1 - xr1) * rdx);
-    const float d2 = fmaf(yb2 - yt2, rdy, (xl2 - xr2) * rdx);
+    const float d0 = fmaf(yn0 - ys0, rdy, (xl0 - xr0) * rdx);
+    const float d1 = fmaf(yn1 - ys1, rdy, (xl1 - xr1) * rdx);
+    const float d2 = fmaf(yn2 - ys2, rdy, (xl2 - xr2) * rdx);
     const float hs = fmaf(dtrate, mult_j, fmaf(-dt, d0, cur.s.h));
     const float hus = fmaf(-dt, d1, cur.hu);
     const float hvs = fmaf(-dt, d2, cur.hv);
@@ -500,33 +522,66 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
         nf_probe = fmaf(h_o + hu_o + hv_o, 0.0f, nf_probe);
       }
     }
-    yt0 = gy.f0;
-    yt1 = gy.ft;
-    yt2 = gy.fn + gy.cB;
+    yc0 = gy.f0;
+    yc1 = gy.ft;
+    yc2 = gy.fn + (SOUTH ? gy.cA : gy.cB);
   };
 
   // Three rows per trip with the carried values rotating through three sets
   // of registers.  A row's raw inputs stay live over three rows (loaded two
-  // rows ahead, read as the north neighbour, then as the current row), so
-  // with two sets the compiler would copy one set at every loop edge, and
-  // wait there for the loads in flight.
+  // rows ahead, read as the next row, then as the current row), so with two
+  // sets the compiler would copy one set at every loop edge, and wait there
+  // for the loads in flight.
   RowCell c1, c2;
   RawCell r1, r2;
   float nn1 = 0.0f, mult1 = 0.0f, nn2 = 0.0f, mult2 = 0.0f;
-  int j = j0;
-  for (; j + 2 < j1; j += 3) {
+  const int n = j1 - j0;
+  int k = 0;
+  for (; k + 2 < n; k += 3) {
+    const int j = jf + k * dj;
     row(j, cur, c1, next, r1, nn, mult, nn1, mult1);
-    row(j + 1, c1, c2, r1, r2, nn1, mult1, nn2, mult2);
-    row(j + 2, c2, cur, r2, next, nn2, mult2, nn, mult);
+    row(j + dj, c1, c2, r1, r2, nn1, mult1, nn2, mult2);
+    row(j + 2 * dj, c2, cur, r2, next, nn2, mult2, nn, mult);
   }
-  if (j < j1) row(j, cur, c1, next, r1, nn, mult, nn1, mult1);
-  if (j + 1 < j1) row(j + 1, c1, c2, r1, r2, nn1, mult1, nn2, mult2);
-  // yt0 now holds the mass flux through the north face of row j1-1.
-  if (LOOP && j1 == G.ny && own) acc.bnd -= (double)yt0 * dxd;
+  if (k < n) row(jf + k * dj, cur, c1, next, r1, nn, mult, nn1, mult1);
+  if (k + 1 < n) row(jf + (k + 1) * dj, c1, c2, r1, r2, nn1, mult1, nn2, mult2);
+  // yc0 now holds the mass flux through the last face of the march: the north
+  // face of row j1-1, or (SOUTH) the south face of row j0.
+  if constexpr (!SOUTH) {
+    if (LOOP && j1 == G.ny && own) acc.bnd -= (double)yc0 * dxd;
+  } else {
+    if (LOOP && j0 == 0 && own) acc.bnd += (double)yc0 * dxd;
+  }
   if (LOOP) {
     acc.nf |= !isfinite(nf_probe);
     acc.clip += (double)clip32;
   }
+}
+
+// One wavefront's strip (see march_strip_dir).  Strips of odd row bands march
+// south and the others north, so two strips that share halo rows (the last
+// two rows of one band and the first two of the next) read them near the
+// same time -- both at the start or both at the end of their marches -- and
+// the second read is an L2 hit.  The direction does not change any cell's
+// arithmetic: every face takes the same operands, and a cell combines its
+// faces in the same order (north minus south, east minus west).
+template <bool LOOP, bool PACKED = false>
+__device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __restrict__ H,
+                                                 const float* __restrict__ HU,
+                                                 const float* __restrict__ HV,
+                                                 float* __restrict__ Ho, float* __restrict__ HUo,
+                                                 float* __restrict__ HVo, int sx, int sy, int rows,
+                                                 float dt, float dtrate, float dtg, double dxd,
+                                                 double dyd, int rec_lo, int rec_hi,
+                                                 float* __restrict__ cube_h,
+                                                 float* __restrict__ cube_hu,
+                                                 float* __restrict__ cube_hv, StripAcc& acc) {
+  if (sy & 1)
+    march_strip_dir<LOOP, PACKED, true>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dt, dtrate, dtg,
+                                        dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
+  else
+    march_strip_dir<LOOP, PACKED, false>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dt, dtrate, dtg,
+                                         dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
 }
 
 }  // namespace swe
