@@ -77,7 +77,7 @@ __device__ __forceinline__ int find_member(const Member* __restrict__ mem, int n
 // layout of the single-domain fast loop (swe_fast.hpp), same arithmetic.
 template <class M, bool PACKED = false>
 __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict__ mem,
-                                                      int n_members, int total_waves,
+                                                      int n_members, int total_waves, int rev,
                                                       EnsPartials P) {
   const int lane = threadIdx.x & (kWave - 1);
   // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so the
@@ -93,7 +93,11 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
   const int w_lo = (blockIdx.x % kXcd) * chunk;
   const int w_hi = min(w_lo + chunk, total_waves);
   const int stride = per_xcd * wpb;
-  for (int w = w_lo + (blockIdx.x / kXcd) * wpb + wave_in_block(); w < w_hi; w += stride) {
+  for (int wd = w_lo + (blockIdx.x / kXcd) * wpb + wave_in_block(); wd < w_hi; wd += stride) {
+    // rev: odd launches sweep the strip list backwards, so each launch starts
+    // on the strips the previous one touched last (still in the memory-side
+    // cache).  The order does not change any cell's arithmetic.
+    const int w = rev ? total_waves - 1 - wd : wd;
     const int m = find_member(mem, n_members, w);
     const Member& mb = mem[m];
     const LoopControl* ctl = mb.ctl;
@@ -253,27 +257,27 @@ int layout(SweEnsemble* e, const std::vector<char>* running = nullptr) {
   return SWE_OK;
 }
 
-void launch_ens(SweEnsemble* e, hipStream_t s) {
+void launch_ens(SweEnsemble* e, int rev, hipStream_t s) {
   if (e->variant == SWE_VARIANT_REFERENCE)
     hipLaunchKernelGGL(k_ensemble_step<RefMath>, dim3(e->grid), dim3(256), e->step_lds, s, e->d_mem,
-                       (int)e->act.size(), e->total_waves, e->P);
+                       (int)e->act.size(), e->total_waves, rev, e->P);
   else
     hipLaunchKernelGGL((k_ensemble_step<FastMath, true>), dim3(e->grid), dim3(256), e->step_lds, s,
-                       e->d_mem, (int)e->act.size(), e->total_waves, e->P);
+                       e->d_mem, (int)e->act.size(), e->total_waves, rev, e->P);
   hipLaunchKernelGGL(k_ensemble_finish, dim3((unsigned)e->act.size()), dim3(256), 0, s, e->d_mem,
                      e->P);
 }
 
 int launch_chunks(SweEnsemble* e, int launches, hipStream_t s) {
   if (e->graph_chunk <= 0) {
-    for (int k = 0; k < launches; ++k) launch_ens(e, s);
+    for (int k = 0; k < launches; ++k) launch_ens(e, k & 1, s);
     ENS_TRY(hipGetLastError());
     return SWE_OK;
   }
   if (!e->graph) {
     if (!e->cap_stream) ENS_TRY(hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking));
     ENS_TRY(hipStreamBeginCapture(e->cap_stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < e->graph_chunk; ++k) launch_ens(e, e->cap_stream);
+    for (int k = 0; k < e->graph_chunk; ++k) launch_ens(e, k & 1, e->cap_stream);
     hipGraph_t g = nullptr;
     ENS_TRY(hipStreamEndCapture(e->cap_stream, &g));
     hipError_t err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);

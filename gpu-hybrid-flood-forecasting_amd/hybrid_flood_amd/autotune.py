@@ -107,11 +107,15 @@ def time_loop_rows(ctx, state: SWEState, rows: int, steps: int) -> float:
 
 
 def tune_rows(params: SWEParams, state: SWEState, *, variant: str | None = None, steps: int = 32,
-              candidates=None, waves=CANDIDATE_WAVES, use_cache: bool = True) -> dict:
+              candidates=None, waves=CANDIDATE_WAVES, use_cache: bool = True, finalists: int = 4,
+              final_steps: int = 256) -> dict:
     """Pick and apply the fastest (strip height, waves per CU) for this grid; returns the record.
 
     ``candidates=None`` searches CANDIDATE_ROWS plus, for each cap, the
-    heights of round_rows; explicit candidates are searched as given."""
+    heights of round_rows; explicit candidates are searched as given.  Every
+    pair is timed over ``steps`` loop steps; the ``finalists`` fastest are
+    then timed again over ``final_steps``, twice, and the best of those wins
+    (short timings alone confuse tiles within a percent of each other)."""
     from .numerics import DEFAULT_VARIANT
 
     variant = variant or DEFAULT_VARIANT
@@ -130,9 +134,18 @@ def tune_rows(params: SWEParams, state: SWEState, *, variant: str | None = None,
                     if candidates is None else [int(r) for r in candidates])
             for r in rows:
                 timings[(int(r), int(w))] = time_loop_rows(ctx, state, int(r), chunk_steps)
-        best = min(timings, key=timings.get)
-        rec = {"rows": best[0], "waves_per_cu": best[1], "ms_per_step": timings[best],
-               "candidates": {f"{r}x{w}": v for (r, w), v in timings.items()}}
+        final = {}
+        if finalists > 1 and len(timings) > 1:
+            long_steps = max(2, final_steps - final_steps % 2)
+            ctx.set_graph_chunk(min(long_steps, 64))
+            for key in sorted(timings, key=timings.get)[:finalists]:
+                ctx.set_waves_per_cu(key[1])
+                final[key] = min(time_loop_rows(ctx, state, key[0], long_steps) for _ in range(2))
+        pool = final or timings
+        best = min(pool, key=pool.get)
+        rec = {"rows": best[0], "waves_per_cu": best[1], "ms_per_step": pool[best],
+               "candidates": {f"{r}x{w}": v for (r, w), v in timings.items()},
+               "finalists": {f"{r}x{w}": v for (r, w), v in final.items()}}
         if use_cache:
             cache[key] = rec
             _store_cache(cache)

@@ -63,6 +63,10 @@ def test_tune_rows_records_and_applies(cuda, tmp_path, monkeypatch):
                              waves=(0, 16))
     assert rec["rows"] in (8, 16) and rec["waves_per_cu"] in (0, 16)
     assert len(rec["candidates"]) == 4
+    # the winner comes from the longer re-timing of the fastest candidates
+    assert len(rec["finalists"]) == 4
+    assert rec["ms_per_step"] == min(rec["finalists"].values())
+    assert rec["ms_per_step"] == rec["finalists"][f"{rec['rows']}x{rec['waves_per_cu']}"]
     assert (tmp_path / "tune.json").exists()
     again = autotune.tune_rows(params, _state(case, cuda), steps=8, candidates=(8, 16),
                                waves=(0, 16))
