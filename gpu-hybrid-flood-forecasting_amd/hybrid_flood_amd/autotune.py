@@ -138,9 +138,9 @@ def tune_rows(params: SWEParams, state: SWEState, *, variant: str | None = None,
         if finalists > 1 and len(timings) > 1:
             long_steps = max(2, final_steps - final_steps % 2)
             ctx.set_graph_chunk(min(long_steps, 64))
-            for key in sorted(timings, key=timings.get)[:finalists]:
-                ctx.set_waves_per_cu(key[1])
-                final[key] = min(time_loop_rows(ctx, state, key[0], long_steps) for _ in range(2))
+            for tile in sorted(timings, key=timings.get)[:finalists]:
+                ctx.set_waves_per_cu(tile[1])
+                final[tile] = min(time_loop_rows(ctx, state, tile[0], long_steps) for _ in range(2))
         pool = final or timings
         best = min(pool, key=pool.get)
         rec = {"rows": best[0], "waves_per_cu": best[1], "ms_per_step": pool[best],
