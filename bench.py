@@ -293,6 +293,79 @@ def bench_adjoint(variant: str, n: int, steps: int, plain_cups: float):
     }
 
 
+# Algorithmic bytes per cell of the path's callers (what each kernel must read
+# and write once): features 12 state + 1 mask + 8 static in, 24 out;
+# correction = sums pass (h, mask, the NHWC residual's line) 17 + apply pass
+# (state, mask, residual) 25 in, 12 out; staging 12 + 1 + 8 in, 24 out;
+# depth correction 4 + 4 in, 4 out; output schema 12 + 1 in, 16 out.
+CALLER_BYTES = {"network_features": 45, "apply_residual_correction": 54, "stage_features": 45,
+                "correct_depth": 12, "conserved_to_primitive": 29}
+
+
+def bench_callers(params, case, reps: int = 20) -> dict:
+    """The step's callers at the roofline grid: the coupled forecast's
+    features and residual correction, batched staging, depth correction and
+    the output schema.  Each op runs `reps` times back to back between HIP
+    events on torch's current stream (the stream the library launches on);
+    the time includes the Python call overhead between launches."""
+    import numpy as np
+    import torch
+
+    from hybrid_flood_amd import coupled_forecast as cf
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd import staging as stg
+    from hybrid_flood_amd import structured_io as sio
+
+    dev = params.bed.device
+    ny, nx = params.bed.shape
+    cells = ny * nx
+    h = torch.as_tensor(case.h0, device=dev).contiguous()
+    state = nm.SWEState(h, (0.1 * h).contiguous(), (-0.05 * h).contiguous())
+    bed = params.bed
+    terrain = ((bed - bed.mean()) / bed.std()).contiguous()
+    rough = ((params.manning_n - params.manning_n.mean()) / params.manning_n.std()).contiguous()
+    hi = cf.HybridInputs(terrain_standardized=terrain, roughness_standardized=rough,
+                         state_mean=(0.1, 0.0, 0.0), state_std=(0.2, 0.05, 0.05),
+                         rainfall_scale_mm_hr=50.0, residual_std=(0.01, 0.01, 0.01))
+    feats = torch.empty((1, ny, nx, 6), dtype=torch.float32, device=dev)
+    g = torch.Generator(device=dev).manual_seed(7)
+    resid = 1e-3 * torch.randn((1, ny, nx, 3), generator=g, device=dev)
+    out = nm.SWEState(*(torch.empty_like(h) for _ in range(3)))
+    prim = torch.stack([h, 0.1 * torch.ones_like(h), torch.zeros_like(h)], dim=-1)[None].contiguous()
+    rain = np.array([5.0], np.float32)
+    pred = 1e-3 * torch.randn((1, 3, ny, nx), generator=g, device=dev)
+    depth = h[None].contiguous()
+    ops = {
+        "network_features": lambda: cf.network_features(state, params, hi, 600.0, out=feats),
+        "apply_residual_correction": lambda: cf.apply_residual_correction(
+            state, resid, params, correction_relaxation=0.5, out=out),
+        "stage_features": lambda: stg.stage_features(prim, rain, terrain, rough,
+                                                     params.domain_mask, (0.1, 0.0, 0.0),
+                                                     (0.2, 0.05, 0.05), 50.0),
+        "correct_depth": lambda: stg.correct_depth(depth, pred),
+        "conserved_to_primitive": lambda: sio._primitive(params, h, state.hu, state.hv),
+    }
+    res = {"grid": f"{ny}x{nx}", "reps": reps}
+    for name, op in ops.items():
+        op()
+        op()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            op()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = CALLER_BYTES[name] * cells / (ms * 1e-3) / 1e9
+        res[name] = {"api_ms": ms, "bytes_per_cell": CALLER_BYTES[name], "api_GB_s": gbs,
+                     "api_frac_of_8TBs": gbs / HBM_PEAK_GBS}
+    res["note"] = ("api_* = per call of the Python API, launch overhead included; the kernels' own "
+                   "durations are in profiles/callers_4096_kernel_stats.md (rocprofv3)")
+    return res
+
+
 def copy_calibration() -> int:
     """Four device-to-device copies of 256 MiB each: 256 MiB read and 256 MiB
     written per dispatch, the reference for the per-dispatch HBM byte counters."""
@@ -324,6 +397,8 @@ def main(argv=None) -> int:
     ap.add_argument("--ensemble-t", type=float, default=3600.0, help="simulated seconds per member")
     ap.add_argument("--adjoint-n", type=int, default=1024, help="adjoint grid side (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--callers", type=int, default=1,
+                    help="time the coupled-forecast, staging and output kernels (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 device copies of 256 MiB (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -408,6 +483,8 @@ def main(argv=None) -> int:
         line["ensemble"] = bench_ensemble(args.variant, args.ensemble, 42, args.ensemble_t, value)
     if args.adjoint_n > 0:
         line["adjoint"] = bench_adjoint(args.variant, args.adjoint_n, args.adjoint_steps, value)
+    if args.callers > 0:
+        line["callers"] = bench_callers(params, case)
     if args.cpu_steps > 0:
         line["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -243,3 +243,56 @@ def test_stage_features_and_correct_depth_bit_exact(cuda):
     out = correct_depth(depth, pred)
     assert np.array_equal(out.cpu().numpy(),
                           co.correct_depth(depth, pred.permute(0, 2, 3, 1).cpu().numpy()))
+
+
+def _flat_case(ny, nx, seed):
+    """Random state and fields without the terrain generator (fast at 1M cells)."""
+    rng = np.random.default_rng(seed)
+    h = np.abs(rng.normal(0, 0.3, (ny, nx))).astype(np.float32)
+    h[rng.uniform(size=(ny, nx)) < 0.1] = 0.0
+    return dict(bed=rng.normal(0, 1, (ny, nx)).astype(np.float32),
+                manning_n=np.full((ny, nx), 0.03, np.float32),
+                domain_mask=rng.uniform(size=(ny, nx)) > 0.1,
+                rainfall_multiplier=None, boundary_types=(0, 0, 0, 0),
+                rain_t=np.array([0.0, 100.0], np.float32), rain_r=np.array([1e-5, 2e-5], np.float32),
+                dx=2.0, dry=1e-4, h=h, hu=(h * rng.normal(0, 1, (ny, nx))).astype(np.float32),
+                hv=(h * rng.normal(0, 1, (ny, nx))).astype(np.float32))
+
+
+@pytest.mark.parametrize("layout", ["nhwc", "nchw"])
+def test_features_bit_exact_past_one_grid_sweep(cuda, layout):
+    """An odd-sized grid above 4096 x 256 cells: every workgroup loops over
+    several tiles, and the NHWC path stages its tiles through LDS."""
+    from hybrid_flood_amd.coupled_forecast import network_features
+    from oracle import coupled_oracle as co
+    from oracle import swe_oracle as o
+
+    case = _flat_case(1031, 1029, 21)
+    params = _params(case, cuda)
+    hi, terrain, rough = _hybrid_inputs(case, cuda, seed=5)
+    got = network_features(_state(case, cuda), params, hi, 30.0, layout=layout).cpu().numpy()
+    if layout == "nchw":
+        got = got.transpose(0, 2, 3, 1)
+    want = co.network_features(o.OracleState(case["h"], case["hu"], case["hv"]), _oparams(case),
+                               terrain, rough, hi.state_mean, hi.state_std, 12.0, 30.0)
+    assert np.array_equal(got, want)
+
+
+def test_stage_features_bit_exact_on_a_large_odd_batch(cuda):
+    """Two samples of an odd cell count: the second sample's output starts
+    8 bytes off a 16-byte boundary and takes the strided path."""
+    import torch
+
+    from hybrid_flood_amd.staging import stage_features
+    from oracle import coupled_oracle as co
+
+    rng = np.random.default_rng(17)
+    count, ny, nx = 2, 1031, 1029
+    st = rng.normal(0, 1, (count, ny, nx, 3)).astype(np.float32)
+    mask = rng.uniform(size=(ny, nx)) > 0.2
+    terrain, rough = rng.normal(size=(2, ny, nx)).astype(np.float32)
+    rain = np.array([3.0, 17.0], np.float32)
+    got = stage_features(torch.as_tensor(st, device=cuda), rain, terrain, rough, mask,
+                         [0.1, -0.2, 0.05], [0.3, 0.7, 0.9], 25.0)
+    want = co.stage_features(st, rain, terrain, rough, mask, [0.1, -0.2, 0.05], [0.3, 0.7, 0.9], 25.0)
+    assert np.array_equal(got.permute(0, 2, 3, 1).cpu().numpy(), want)
