@@ -1,10 +1,11 @@
 // Device building blocks of the fused shallow-water step for gfx950.
 //
 // One wavefront owns a strip of 64 consecutive columns (62 updated, one halo
-// column each side) and marches north through a band of rows.  The cell to
-// the west comes from the neighbouring lane by a DPP wave shift, the row to
-// the north is the next row's load, so every face flux is evaluated once per
-// strip and no LDS or barrier is needed inside the march.
+// column each side) and marches through a band of rows (north; in the fast
+// variant, odd bands march south).  The cell to the west comes from the
+// neighbouring lane by a DPP wave shift, the next row is the next row's load,
+// so every face flux is evaluated once per strip and no LDS or barrier is
+// needed inside the march.
 //
 // The arithmetic is the reference step (jax_solver/numerics.py:58-369) in
 // face-normal coordinates.  Two policies evaluate it:
