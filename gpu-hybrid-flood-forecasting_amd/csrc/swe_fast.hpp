@@ -413,10 +413,6 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     // sign of row j's n' (one compare, straight into a lane mask) rather than
     // carried over from the previous row, which the compiler does through a
     // vector register and two conversions.
-    // The empty asm keeps the compiler from folding the compare into the
-    // loop's phi nodes, which would move it to the loop edge and make the
-    // edge wait for the loads in flight.
-    if constexpr (PACKED) asm("" : "+v"(nn_j));
     const bool ca = PACKED ? !signbit(nn_j) : cur.s.a;
 
     // x face between lane-1 (A) and this lane (B).  Every face of lanes
