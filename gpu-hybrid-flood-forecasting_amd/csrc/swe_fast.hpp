@@ -357,9 +357,11 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   // (j1-1 .. j0); dj is the step.  "Ahead" below means the march direction.
   constexpr int dj = SOUTH ? -1 : 1;
   const int jf = SOUTH ? j1 - 1 : j0;  // first row of the march
-  // Prologue: the face behind row jf, rows jf (current) and jf+dj (in flight).
+  // Prologue: the face behind row jf, rows jf (current), jf+dj and jf+2dj
+  // (in flight).
   RowCell cur = side_of(load_raw(G, io, jf - dj), neg_hu, dry, G.half_g);
   RawCell next = load_raw(G, io, jf);
+  RawCell r2;
   float yc0, yc1, yc2;  // carried y face: between the current row and the one behind it
   float nn, mult;
   {
@@ -369,6 +371,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
       mult = next.m;
     }
     next = load_raw(G, io, jf + dj);
+    r2 = load_raw(G, io, SOUTH ? max(jf - 2, j0 - 1) : min(jf + 2, j1));
     if constexpr (!SOUTH) {  // south face of row j0
       const FaceFlux fy = face_fast_any<false>(G, cur.s, c0.s, wave_all(cur.s.a && c0.s.a));
       if (LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
@@ -392,14 +395,14 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   float clip32 = 0.0f;  // clip depth is small; float32 per lane over the strip
 
   // Row j: `cur` is row j, `north` row j+dj (the next row of the march,
-  // loaded one row ago).  Issues the loads of row j+2dj into `ahead` and of
-  // row j+dj's n/multiplier into nn_next/mult_next, and leaves row j+dj's
-  // side in `up`.
+  // loaded two rows ago; row j+2dj is in flight too).  Issues the loads of
+  // row j+3dj into `ahead`, takes row j+dj's n/multiplier into
+  // nn_next/mult_next, and leaves row j+dj's side in `up`.
   auto row = [&](int j, const RowCell& cur, RowCell& up, const RawCell& north, RawCell& ahead,
                  float nn_j, float mult_j, float& nn_next, float& mult_next) {
     // Unconditional, so the wait counts stay exact; past the strip the
     // clamped rows were just loaded and come from the cache.
-    ahead = load_raw(G, io, SOUTH ? max(j - 2, j0 - 1) : min(j + 2, j1));
+    ahead = load_raw(G, io, SOUTH ? max(j - 3, j0 - 1) : min(j + 3, j1));
     if constexpr (PACKED) {  // row j+dj's fields arrived with its state
       nn_next = north.n;
       mult_next = north.m;
@@ -523,24 +526,29 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     yc2 = gy.fn + (SOUTH ? gy.cA : gy.cB);
   };
 
-  // Three rows per trip with the carried values rotating through three sets
-  // of registers.  A row's raw inputs stay live over three rows (loaded two
-  // rows ahead, read as the next row, then as the current row), so with two
-  // sets the compiler would copy one set at every loop edge, and wait there
-  // for the loads in flight.
-  RowCell c1, c2;
-  RawCell r1, r2;
-  float nn1 = 0.0f, mult1 = 0.0f, nn2 = 0.0f, mult2 = 0.0f;
+  // Four rows per trip with the raw rows rotating through four sets of
+  // registers: a row's raw inputs stay live over four rows (loaded three
+  // rows ahead, read as the next row, then as the current row), so fewer
+  // sets would make the compiler copy one at every loop edge, and wait there
+  // for the loads in flight.  The derived row values alternate between two.
+  // The raw sets, by the row (relative to the trip's first row j) they hold
+  // when row j starts: next = j+1, r2 = j+2 (both from the prologue or the
+  // previous trip), r3 and r1 are filled with rows j+3 and j+4 on the way.
+  RowCell c1;
+  RawCell r1, r3;
+  float nn1 = 0.0f, mult1 = 0.0f;
   const int n = j1 - j0;
   int k = 0;
-  for (; k + 2 < n; k += 3) {
+  for (; k + 3 < n; k += 4) {
     const int j = jf + k * dj;
-    row(j, cur, c1, next, r1, nn, mult, nn1, mult1);
-    row(j + dj, c1, c2, r1, r2, nn1, mult1, nn2, mult2);
-    row(j + 2 * dj, c2, cur, r2, next, nn2, mult2, nn, mult);
+    row(j, cur, c1, next, r3, nn, mult, nn1, mult1);
+    row(j + dj, c1, cur, r2, r1, nn1, mult1, nn, mult);
+    row(j + 2 * dj, cur, c1, r3, next, nn, mult, nn1, mult1);
+    row(j + 3 * dj, c1, cur, r1, r2, nn1, mult1, nn, mult);
   }
-  if (k < n) row(jf + k * dj, cur, c1, next, r1, nn, mult, nn1, mult1);
-  if (k + 1 < n) row(jf + (k + 1) * dj, c1, c2, r1, r2, nn1, mult1, nn2, mult2);
+  if (k < n) row(jf + k * dj, cur, c1, next, r3, nn, mult, nn1, mult1);
+  if (k + 1 < n) row(jf + (k + 1) * dj, c1, cur, r2, r1, nn1, mult1, nn, mult);
+  if (k + 2 < n) row(jf + (k + 2) * dj, cur, c1, r3, next, nn, mult, nn1, mult1);
   // yc0 now holds the mass flux through the last face of the march: the north
   // face of row j1-1, or (SOUTH) the south face of row j0.
   if constexpr (!SOUTH) {
