@@ -433,7 +433,8 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
     rain_total += 2 * (size_t)members[m].params.n_rainfall;
     cells_total += (size_t)members[m].params.nx * members[m].params.ny;
   }
-  if (hipMalloc(&e->d_rain, sizeof(double) * rain_total) != hipSuccess ||
+  // d_rain holds every member's rain series, then one FP64 multiplier sum per member.
+  if (hipMalloc(&e->d_rain, sizeof(double) * (rain_total + n)) != hipSuccess ||
       hipMalloc(&e->d_times, sizeof(double) * 2 * n) != hipSuccess ||
       hipMalloc(&e->d_ctl, sizeof(LoopControl) * n) != hipSuccess ||
       hipHostMalloc(&e->h_ctl, sizeof(LoopControl) * n, hipHostMallocDefault) != hipSuccess ||
@@ -490,7 +491,9 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
   }
   if (hipMemcpy(e->d_rain, rain_all.data(), sizeof(double) * rain_all.size(), hipMemcpyHostToDevice) != hipSuccess)
     return bail(efail(SWE_EHIP, "swe_ensemble_create: upload rain"));
-  // FP64 rain-multiplier sums over active cells, one member at a time (setup only).
+  // FP64 rain-multiplier sums over active cells, one member at a time, taken
+  // at creation (the rain volume of a member's diagnostics uses them).
+  std::vector<double> sums(n);
   for (int m = 0; m < n; ++m) {
     const SweParamsDesc& p = members[m].params;
     const size_t N = (size_t)p.nx * p.ny;
@@ -502,8 +505,11 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
     double s = 0.0;
     for (size_t k = 0; k < N; ++k)
       if (mask[k]) s += (double)mult[k];
-    e->host[m].L.sum_mult = s;
+    sums[m] = s;
+    e->host[m].L.sum_mult = e->d_rain + rain_total + m;
   }
+  if (hipMemcpy(e->d_rain + rain_total, sums.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+    return bail(efail(SWE_EHIP, "swe_ensemble_create: upload multiplier sums"));
   int rc = layout(e);
   if (rc) return bail(rc);
   *out = e;

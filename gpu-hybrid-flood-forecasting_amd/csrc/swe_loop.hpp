@@ -41,7 +41,10 @@ struct LoopControl {
 };
 
 struct LoopConst {
-  double cfl, max_dt, gravity, cell_area, sum_mult, rain_default, dxd, dyd;
+  double cfl, max_dt, gravity, cell_area, rain_default, dxd, dyd;
+  // FP64 sum of the rain multiplier over active cells (device-resident, so a
+  // loop start can refresh it without a host round trip).
+  const double* sum_mult;
   const double* rain_t;
   const double* rain_r;
   const double* times;
@@ -199,7 +202,7 @@ __device__ int rain_bracket(const LoopConst& L, double t) {
 // step.  Resets the workgroup counter for the next launch.
 __device__ inline void finish_step(LoopControl* ctl, const LoopConst& L, const Totals& tot,
                                    double t_new, int rain_j) {
-  ctl->rain += (double)ctl->dtrate_f * L.sum_mult * L.cell_area;
+  ctl->rain += (double)ctl->dtrate_f * *L.sum_mult * L.cell_area;
   ctl->bnd += ctl->dt * tot.bnd;
   ctl->clip += tot.clip * L.cell_area;
   ctl->vol = tot.vol * L.cell_area;

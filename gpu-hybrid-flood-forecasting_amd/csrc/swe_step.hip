@@ -192,6 +192,15 @@ __global__ __launch_bounds__(kBlock) void k_sum_mult(Grid G, double* __restrict_
   if (threadIdx.x == 0) partial[blockIdx.x] = t.vol;
 }
 
+// One workgroup: out = the sum of partial[0 .. n) in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partial, int n,
+                                                         double* __restrict__ out) {
+  double s = 0.0;
+  for (int k = threadIdx.x; k < n; k += kBlock) s += partial[k];
+  const Totals t = block_reduce(0.0f, s, 0.0, 0.0, 0);
+  if (threadIdx.x == 0) *out = t.vol;
+}
+
 // Copy the loop's final state (buffer A or B by the parity of the step count)
 // to `dst`, so the host need not know how many steps the device took.
 __global__ __launch_bounds__(kBlock) void k_fill3(float* __restrict__ a, float* __restrict__ b,
@@ -295,7 +304,7 @@ struct SweContext {
   int graph_chunk = 64;
   double dx = 1, dy = 1, gravity = 9.81, cfl = 0.45, dry = 1e-4, max_dt = 10, rain_default = 0;
   std::vector<double> rain_t, rain_r;
-  double sum_mult = 0.0;
+  double* d_sum_mult = nullptr;  // FP64 rain-multiplier sum over active cells
   double* d_rain = nullptr;  // [rain_t | rain_r]
   double* d_times = nullptr;
   int times_cap = 0;
@@ -345,7 +354,7 @@ struct SweContext {
     L.max_dt = max_dt;
     L.gravity = gravity;
     L.cell_area = dx * dy;
-    L.sum_mult = sum_mult;
+    L.sum_mult = d_sum_mult;
     L.rain_default = rain_default;
     L.dxd = dx;
     L.dyd = dy;
@@ -414,7 +423,19 @@ int ensure_work(SweContext* c) {
 // Start a device loop on `src`: pick the work layout for the variant and fill
 // work state 0 (for the packed layout also the packed static fields, from the
 // caller's current arrays).
+// The FP64 sum of the rain multiplier over active cells into d_sum_mult (the
+// fields may have been changed in place since the last run).  Uses the
+// partials as scratch.
+int refresh_sum_mult(SweContext* c, hipStream_t s) {
+  const int nb = c->reduce_blocks();
+  hipLaunchKernelGGL(k_sum_mult, dim3(nb), dim3(kBlock), 0, s, c->G, c->P.vol);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, c->P.vol, nb, c->d_sum_mult);
+  SWE_HIP_TRY(hipGetLastError());
+  return SWE_OK;
+}
+
 int stage_in(SweContext* c, const SweState& src, hipStream_t s) {
+  if (int rc = refresh_sum_mult(c, s)) return rc;
   c->loop_packed = c->packed_variant();
   if (!c->loop_packed) return copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), src.h, src.hu, src.hv, s);
   const size_t N = c->cells();
@@ -794,17 +815,13 @@ int swe_create(const SweParamsDesc* p, int32_t variant, SweContext** out) {
   c->rows = default_rows(G.nx, G.ny);
   int rc = ensure_partials(c);
   if (rc) return cleanup(rc);
-  // Sum of the rain multiplier over active cells, for the FP64 rain volume.
-  {
-    const int nb = c->reduce_blocks();
-    hipLaunchKernelGGL(k_sum_mult, dim3(nb), dim3(kBlock), 0, 0, c->G, c->P.vol);
-    std::vector<double> part(nb);
-    if (hipMemcpy(part.data(), c->P.vol, sizeof(double) * nb, hipMemcpyDeviceToHost) != hipSuccess)
-      return cleanup(fail(SWE_EHIP, "swe_create: rain multiplier sum"));
-    double s = 0.0;
-    for (double v : part) s += v;
-    c->sum_mult = s;
-  }
+  // Sum of the rain multiplier over active cells, for the FP64 rain volume
+  // (refreshed at every loop start).
+  if (hipMalloc(&c->d_sum_mult, sizeof(double)) != hipSuccess)
+    return cleanup(fail(SWE_EHIP, "swe_create: hipMalloc rain multiplier sum"));
+  if ((rc = refresh_sum_mult(c, nullptr))) return cleanup(rc);
+  if (hipDeviceSynchronize() != hipSuccess)
+    return cleanup(fail(SWE_EHIP, "swe_create: rain multiplier sum"));
   *out = c;
   return SWE_OK;
 }
@@ -818,6 +835,7 @@ int swe_destroy(SweContext* c) {
   if (c->P.vol) (void)hipFree(c->P.vol);
   if (c->d_times) (void)hipFree(c->d_times);
   if (c->d_ctl) (void)hipFree(c->d_ctl);
+  if (c->d_sum_mult) (void)hipFree(c->d_sum_mult);
   if (c->h_ctl) (void)hipHostFree(c->h_ctl);
   if (c->d_rain) (void)hipFree(c->d_rain);
   if (c->aux) (void)hipFree(c->aux);

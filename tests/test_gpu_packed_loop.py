@@ -102,6 +102,13 @@ def test_fields_changed_in_place_are_used(cuda):
     for a, b in zip(second.states, fresh.states):
         assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
     assert not np.array_equal(first.states.h[-1].cpu().numpy(), second.states.h[-1].cpu().numpy())
+    # the FP64 rain volume follows the changed multiplier too (its sum is
+    # taken at every run start), so the mass balance still closes
+    assert second.diagnostics["rain_volume"] == pytest.approx(fresh.diagnostics["rain_volume"],
+                                                              rel=1e-12)
+    assert second.diagnostics["rain_volume"] != pytest.approx(first.diagnostics["rain_volume"],
+                                                              rel=1e-3)
+    assert abs(second.diagnostics["mass_balance_rel"]) < 1e-6
 
 
 def test_variant_switch_on_one_context(cuda):
