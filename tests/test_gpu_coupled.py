@@ -217,6 +217,42 @@ def test_result_to_dataset_matches_oracle(cuda, tmp_path):
     assert back["__meta__"]["var_attrs"]["velocity"]["units"] == "m s-1"
 
 
+@pytest.mark.parametrize("shape,offset", [((24, 36), 0), ((24, 36), 1), ((25, 37), 0), ((64, 4), 0)])
+def test_conserved_to_dataset_vector_and_scalar_forms(cuda, shape, offset):
+    """Four-cell form (cell count a multiple of 4, aligned arrays) and the
+    per-cell form (odd counts, or inputs one float off alignment) give the
+    oracle's schema exactly, across several output times."""
+    import torch
+
+    from hybrid_flood_amd.structured_io import conserved_to_dataset
+    from oracle import output_oracle as oo
+
+    case = random_case(16, shape, speed=1.0)
+    params = _params(case, cuda)
+    rng = np.random.default_rng(17)
+    T = 3
+    h, hu, hv = (rng.normal(0.0, 1.0, (3, T) + shape).astype(np.float32) *
+                 np.array([0.05, 0.2, 0.2], np.float32)[:, None, None, None])
+    h = np.abs(h)
+    h[:, ::3, ::2] = 0.5 * np.float32(case["dry"])  # dry cells with momentum: zero velocity
+
+    def dev(a):
+        buf = torch.empty(a.size + offset, dtype=torch.float32, device=cuda)
+        buf[offset:] = torch.as_tensor(a.ravel(), device=cuda)
+        return buf[offset:].view(a.shape)
+
+    ds = conserved_to_dataset(dev(h), dev(hu), dev(hv), np.arange(T, dtype=np.float32), params,
+                              dry_tolerance_m=float(case["dry"]), title="t", backend="hip",
+                              steps_used=0)
+    want = oo.conserved_to_primitive(h, hu, hv, case["bed"], case["domain_mask"],
+                                     np.float32(case["dry"]))
+    for name in ("elevation", "depth", "x_velocity", "y_velocity"):
+        assert np.array_equal(ds[name].cpu().numpy(), want[name], equal_nan=True), name
+    np.testing.assert_allclose(ds["velocity"].cpu().numpy(), want["velocity"], rtol=1.2e-7,
+                               equal_nan=True)
+    assert not np.isnan(ds["depth"].cpu().numpy()[:, case["domain_mask"]]).any()
+
+
 # ---------------------------------------------------------------------------
 # Feature staging (comparison/v2_staging.py)
 # ---------------------------------------------------------------------------
