@@ -207,6 +207,84 @@ __global__ __launch_bounds__(kBlock) void k_corr_apply(Grid G, const float* H, c
   }
 }
 
+// Four-cell form of pass 3: cells 4q..4q+3 per thread, 16-byte loads and
+// stores, the mask as one 32-bit word (pass 1 gains nothing from it).  The host takes them only when the
+// cell count is a multiple of 4, every array is 16-byte aligned and the
+// residual is interleaved (cell stride 3, channel stride 1) or planar (cell
+// stride 1, channel stride a multiple of 4).  Each cell's arithmetic is that
+// of k_corr_apply.
+template <bool INTERLEAVED>
+__device__ __forceinline__ void residual4(const Residual& R, size_t q, float (&c)[3][4]) {
+  const float4* r4 = reinterpret_cast<const float4*>(R.r);
+  if (INTERLEAVED) {
+    const float4 a = r4[3 * q], b = r4[3 * q + 1], d = r4[3 * q + 2];
+    const float v[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, d.x, d.y, d.z, d.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) c[ch][i] = v[3 * i + ch];
+  } else {
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      const float4 a = r4[(size_t)ch * (size_t)(R.cs / 4) + q];
+      c[ch][0] = a.x;
+      c[ch][1] = a.y;
+      c[ch][2] = a.z;
+      c[ch][3] = a.w;
+    }
+  }
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[ch][i] = clip_to(c[ch][i], R.lim[ch]);
+}
+
+__device__ __forceinline__ float4 ld4(const float* p, size_t q) {
+  return reinterpret_cast<const float4*>(p)[q];
+}
+
+// As k_corr_apply, the output may be the input: each thread reads its four
+// cells before writing them, so no pointer is __restrict__.
+template <bool INTERLEAVED>
+__global__ __launch_bounds__(kBlock) void k_corr_apply4(Grid G, const float* H, const float* HU,
+                                                        const float* HV, Residual R,
+                                                        const double* sc, float* Ho, float* HUo,
+                                                        float* HVo) {
+  const size_t N4 = (size_t)G.nx * (size_t)G.ny / 4;
+  const uint32_t* M = reinterpret_cast<const uint32_t*>(G.mask);
+  const double scale = sc[2];
+  const bool use_scaled = sc[3] != 0.0;
+  for (size_t q = (size_t)blockIdx.x * kBlock + threadIdx.x; q < N4;
+       q += (size_t)gridDim.x * kBlock) {
+    const float4 h4 = ld4(H, q), hu4 = ld4(HU, q), hv4 = ld4(HV, q);
+    const float hs[4] = {h4.x, h4.y, h4.z, h4.w};
+    const float hus[4] = {hu4.x, hu4.y, hu4.z, hu4.w};
+    const float hvs[4] = {hv4.x, hv4.y, hv4.z, hv4.w};
+    const uint32_t m = M[q];
+    float c[3][4];
+    residual4<INTERLEAVED>(R, q, c);
+    float oh[4], ou[4], ov[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float h = hs[i];
+      const bool a = ((m >> (8 * i)) & 0xffu) != 0;
+      const float u = safe_velocity(hus[i], h, G.dry);
+      const float v = safe_velocity(hvs[i], h, G.dry);
+      const float p = a ? fmaxf(h + R.relax * c[0][i], 0.0f) : 0.0f;
+      const float ch = use_scaled ? (float)((double)p * scale) : h;
+      const float cu = u + R.relax * c[1][i];
+      const float cv = v + R.relax * c[2][i];
+      const bool wet = a && ch > G.dry;
+      oh[i] = ch;
+      ou[i] = wet ? ch * cu : 0.0f;
+      ov[i] = wet ? ch * cv : 0.0f;
+    }
+    reinterpret_cast<float4*>(Ho)[q] = make_float4(oh[0], oh[1], oh[2], oh[3]);
+    reinterpret_cast<float4*>(HUo)[q] = make_float4(ou[0], ou[1], ou[2], ou[3]);
+    reinterpret_cast<float4*>(HVo)[q] = make_float4(ov[0], ov[1], ov[2], ov[3]);
+  }
+}
+
 // Feature staging of a batch of primitive states (comparison/v2_staging.py:
 // 17-44): sample s, cell k, channel c of the (count, ny, nx, 3) NHWC state is
 // state[(s*N + k)*3 + c]; the output channel c of (s, k) goes to
@@ -318,9 +396,26 @@ int swe_apply_residual_correction(SweContext* c, SweState predicted, const float
   double* part = scratch + 4;
   hipLaunchKernelGGL(k_corr_sums, dim3(nb), dim3(kBlock), 0, s, G, predicted.h, R, part);
   hipLaunchKernelGGL(k_corr_scale, dim3(1), dim3(kBlock), 0, s, part, nb, scratch);
-  hipLaunchKernelGGL(k_corr_apply, dim3(grid_blocks(N)), dim3(kBlock), 0, s, G, predicted.h,
-                     predicted.hu, predicted.hv, R, scratch, corrected.h, corrected.hu,
-                     corrected.hv);
+  // four-cell apply: N % 4 == 0, 16-byte arrays, a 4-byte mask, and an
+  // interleaved (3, 1) or planar (1, 4k) residual
+  auto al = [](const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; };
+  const bool interleaved = channel_stride == 1 && cell_stride == 3;
+  const bool planar = cell_stride == 1 && channel_stride % 4 == 0;
+  const bool vec = N % 4 == 0 && (interleaved || planar) && al(residual, 16) && al(G.mask, 4) &&
+                   al(predicted.h, 16) && al(predicted.hu, 16) && al(predicted.hv, 16) &&
+                   al(corrected.h, 16) && al(corrected.hu, 16) && al(corrected.hv, 16);
+  if (vec && interleaved)
+    hipLaunchKernelGGL(k_corr_apply4<true>, dim3(grid_blocks(N / 4)), dim3(kBlock), 0, s, G,
+                       predicted.h, predicted.hu, predicted.hv, R, scratch, corrected.h,
+                       corrected.hu, corrected.hv);
+  else if (vec)
+    hipLaunchKernelGGL(k_corr_apply4<false>, dim3(grid_blocks(N / 4)), dim3(kBlock), 0, s, G,
+                       predicted.h, predicted.hu, predicted.hv, R, scratch, corrected.h,
+                       corrected.hu, corrected.hv);
+  else
+    hipLaunchKernelGGL(k_corr_apply, dim3(grid_blocks(N)), dim3(kBlock), 0, s, G, predicted.h,
+                       predicted.hu, predicted.hv, R, scratch, corrected.h, corrected.hu,
+                       corrected.hv);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return set_error(SWE_EHIP, std::string("residual correction: ") + hipGetErrorString(e));

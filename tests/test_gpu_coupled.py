@@ -157,6 +157,40 @@ def test_residual_correction_matches_oracle(cuda, inplace):
     assert float(out.h.double().sum()) == pytest.approx(wvols[0], rel=1e-6)
 
 
+@pytest.mark.parametrize("shape,layout", [((40, 52), "nhwc"), ((40, 52), "offset"),
+                                          ((41, 53), "nhwc"), ((41, 53), "planar")])
+def test_residual_correction_four_cell_and_one_cell_forms(cuda, shape, layout):
+    """Interleaved NHWC and planar residuals on a cell count that is a multiple
+    of 4 take the four-cell apply kernel; an odd count or a residual one float
+    off alignment takes the one-cell kernel; every form matches the oracle."""
+    import torch
+
+    from hybrid_flood_amd.coupled_forecast import apply_residual_correction
+    from oracle import coupled_oracle as co
+    from oracle import swe_oracle as o
+
+    case = random_case(19, shape)
+    params = _params(case, cuda)
+    r_hwc = np.random.default_rng(2).normal(0, 3.0, shape + (3,)).astype(np.float32)
+    lim = np.array([1.5, 0.7, 2.0], np.float32)
+    if layout == "planar":
+        residual = torch.as_tensor(r_hwc.transpose(2, 0, 1).copy(), device=cuda).permute(1, 2, 0)
+    elif layout == "offset":
+        buf = torch.empty(r_hwc.size + 1, dtype=torch.float32, device=cuda)
+        buf[1:] = torch.as_tensor(r_hwc.ravel(), device=cuda)
+        residual = buf[1:].view(r_hwc.shape)
+    else:
+        residual = torch.as_tensor(r_hwc, device=cuda)
+    out, vols = apply_residual_correction(_state(case, cuda), residual, params,
+                                          correction_relaxation=0.4, clip_limits=lim, volumes=True)
+    want, wvols = co.apply_residual_correction(o.OracleState(case["h"], case["hu"], case["hv"]),
+                                               r_hwc, _oparams(case), correction_relaxation=0.4,
+                                               clip_limits=lim)
+    np.testing.assert_allclose(vols, wvols, rtol=1e-12)
+    for g, w in zip(out, want):
+        np.testing.assert_allclose(g.cpu().numpy(), w, rtol=2.4e-7, atol=0)
+
+
 def test_coupled_forecast_matches_oracle(cuda):
     import torch
 
