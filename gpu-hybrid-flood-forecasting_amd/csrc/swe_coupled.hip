@@ -104,6 +104,62 @@ __global__ __launch_bounds__(kBlock) void k_features(Grid G, const float* __rest
   }
 }
 
+// Four cells per thread for NHWC features (channel stride 1, cell stride 6)
+// when the cell count is a multiple of 4 and every array is aligned: 16-byte
+// loads, the mask as one 32-bit word, and the block's 24 * kBlock output
+// floats staged in LDS and written as consecutive 16-byte vectors.  Each
+// cell's arithmetic is that of k_features.
+__global__ __launch_bounds__(kBlock) void k_features4(Grid G, const float* __restrict__ H,
+                                                      const float* __restrict__ HU,
+                                                      const float* __restrict__ HV,
+                                                      const float* __restrict__ terrain,
+                                                      const float* __restrict__ rough, Norm nz,
+                                                      float rain_channel, float* __restrict__ out) {
+  __shared__ __align__(16) float tile[kBlock * 24];
+  const size_t N4 = (size_t)G.nx * (size_t)G.ny / 4;
+  const uint32_t* __restrict__ M = reinterpret_cast<const uint32_t*>(G.mask);
+  const float4* __restrict__ H4 = reinterpret_cast<const float4*>(H);
+  const float4* __restrict__ HU4 = reinterpret_cast<const float4*>(HU);
+  const float4* __restrict__ HV4 = reinterpret_cast<const float4*>(HV);
+  const float4* __restrict__ T4 = reinterpret_cast<const float4*>(terrain);
+  const float4* __restrict__ R4 = reinterpret_cast<const float4*>(rough);
+  // Block-uniform loop over tiles of kBlock quads of cells.
+  for (size_t base = (size_t)blockIdx.x * kBlock; base < N4; base += (size_t)gridDim.x * kBlock) {
+    const size_t q = base + threadIdx.x;
+    if (q < N4) {
+      const float4 h4 = H4[q], hu4 = HU4[q], hv4 = HV4[q], t4 = T4[q], r4 = R4[q];
+      const uint32_t m = M[q];
+      const float hs[4] = {h4.x, h4.y, h4.z, h4.w};
+      const float hus[4] = {hu4.x, hu4.y, hu4.z, hu4.w};
+      const float hvs[4] = {hv4.x, hv4.y, hv4.z, hv4.w};
+      const float ts[4] = {t4.x, t4.y, t4.z, t4.w};
+      const float rs[4] = {r4.x, r4.y, r4.z, r4.w};
+      float f[24];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float h = hs[i];
+        const float u = safe_velocity(hus[i], h, G.dry);
+        const float v = safe_velocity(hvs[i], h, G.dry);
+        const bool a = ((m >> (8 * i)) & 0xffu) != 0;
+        f[6 * i + 0] = a ? (h - nz.mean[0]) / nz.std[0] : 0.0f;
+        f[6 * i + 1] = a ? (u - nz.mean[1]) / nz.std[1] : 0.0f;
+        f[6 * i + 2] = a ? (v - nz.mean[2]) / nz.std[2] : 0.0f;
+        f[6 * i + 3] = rain_channel;
+        f[6 * i + 4] = ts[i];
+        f[6 * i + 5] = rs[i];
+      }
+      float4* t = reinterpret_cast<float4*>(tile) + threadIdx.x * 6;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) t[j] = make_float4(f[4 * j], f[4 * j + 1], f[4 * j + 2], f[4 * j + 3]);
+    }
+    __syncthreads();
+    const int nf4 = 6 * (int)min<size_t>(kBlock, N4 - base);  // float4s of this tile
+    float4* dst = reinterpret_cast<float4*>(out) + base * 6;
+    for (int j = threadIdx.x; j < nf4; j += kBlock) dst[j] = reinterpret_cast<const float4*>(tile)[j];
+    __syncthreads();
+  }
+}
+
 struct Residual {
   const float* r;
   long long cs, ps;
@@ -367,9 +423,17 @@ int swe_network_features(SweContext* c, SweState state, const float* terrain_std
   // coupled_forecast.py:65-69 in float32: rate * 3.6e6, then / scale.
   const float mm_hr = (float)context_rate(c, time_s) * 3.6e6f;
   const float rain_channel = mm_hr / (float)rainfall_scale_mm_hr;
-  hipLaunchKernelGGL(k_features, dim3(grid_blocks(N)), dim3(kBlock), 0, (hipStream_t)stream, G,
-                     state.h, state.hu, state.hv, terrain_std, roughness_std, nz, rain_channel,
-                     features, (long long)channel_stride, (long long)cell_stride);
+  auto al = [](const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; };
+  if (channel_stride == 1 && cell_stride == 6 && N % 4 == 0 && al(features, 16) &&
+      al(G.mask, 4) && al(state.h, 16) && al(state.hu, 16) && al(state.hv, 16) &&
+      al(terrain_std, 16) && al(roughness_std, 16))
+    hipLaunchKernelGGL(k_features4, dim3(grid_blocks(N / 4)), dim3(kBlock), 0,
+                       (hipStream_t)stream, G, state.h, state.hu, state.hv, terrain_std,
+                       roughness_std, nz, rain_channel, features);
+  else
+    hipLaunchKernelGGL(k_features, dim3(grid_blocks(N)), dim3(kBlock), 0, (hipStream_t)stream, G,
+                       state.h, state.hu, state.hv, terrain_std, roughness_std, nz, rain_channel,
+                       features, (long long)channel_stride, (long long)cell_stride);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(SWE_EHIP, std::string("k_features: ") + hipGetErrorString(e));
   return SWE_OK;

@@ -112,13 +112,14 @@ def _hybrid_inputs(case, dev, seed=3):
                         [0.1, 0.02, -0.01], [0.2, 0.3, 0.25], 12.0, [0.01, 0.05, 0.04]), terrain, rough
 
 
+@pytest.mark.parametrize("shape", [(21, 37), (24, 44)])  # one-cell; four-cell NHWC form
 @pytest.mark.parametrize("layout", ["nhwc", "nchw"])
-def test_features_bit_exact(cuda, layout):
+def test_features_bit_exact(cuda, layout, shape):
     from hybrid_flood_amd.coupled_forecast import network_features
     from oracle import coupled_oracle as co
     from oracle import swe_oracle as o
 
-    case = random_case(12, (21, 37))
+    case = random_case(12, shape)
     params = _params(case, cuda)
     hi, terrain, rough = _hybrid_inputs(case, cuda)
     got = network_features(_state(case, cuda), params, hi, 75.0, layout=layout).cpu().numpy()
