@@ -379,6 +379,59 @@ __global__ __launch_bounds__(kBlock) void k_stage(size_t N, const float* __restr
   }
 }
 
+// Four cells per thread of k_stage for NHWC output (channel stride 1, cell
+// stride 6) when the cell count and the sample stride are multiples of 4 and
+// every array is aligned: the (depth, u, v) of four cells as three 16-byte
+// loads, the mask as one 32-bit word, and the block's 24 * kBlock output
+// floats staged in LDS and written as 16-byte vectors.  Each cell's
+// arithmetic is that of k_stage.  blockIdx.y is the sample.
+__global__ __launch_bounds__(kBlock) void k_stage4(size_t N, const float* __restrict__ state,
+                                                   const uint8_t* __restrict__ mask,
+                                                   const float* __restrict__ terrain,
+                                                   const float* __restrict__ rough, Norm nz,
+                                                   const float* __restrict__ rain_ch,
+                                                   float* __restrict__ out, long long ss) {
+  __shared__ __align__(16) float tile[kBlock * 24];
+  const size_t smp = blockIdx.y;
+  const size_t N4 = N / 4;
+  const float rain = rain_ch[smp];
+  const float4* __restrict__ S4 = reinterpret_cast<const float4*>(state + smp * N * 3);
+  const uint32_t* __restrict__ M = reinterpret_cast<const uint32_t*>(mask);
+  const float4* __restrict__ T4 = reinterpret_cast<const float4*>(terrain);
+  const float4* __restrict__ R4 = reinterpret_cast<const float4*>(rough);
+  float4* __restrict__ o4 = reinterpret_cast<float4*>(out + (long long)smp * ss);
+  for (size_t base = (size_t)blockIdx.x * kBlock; base < N4; base += (size_t)gridDim.x * kBlock) {
+    const size_t q = base + threadIdx.x;
+    if (q < N4) {
+      const float4 a4 = S4[3 * q], b4 = S4[3 * q + 1], c4 = S4[3 * q + 2];
+      const float sv[12] = {a4.x, a4.y, a4.z, a4.w, b4.x, b4.y, b4.z, b4.w, c4.x, c4.y, c4.z, c4.w};
+      const float4 t4 = T4[q], r4 = R4[q];
+      const float ts[4] = {t4.x, t4.y, t4.z, t4.w};
+      const float rs[4] = {r4.x, r4.y, r4.z, r4.w};
+      const uint32_t m = M[q];
+      float f[24];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const bool a = ((m >> (8 * i)) & 0xffu) != 0;
+        f[6 * i + 0] = a ? (sv[3 * i] - nz.mean[0]) / nz.std[0] : 0.0f;
+        f[6 * i + 1] = a ? (sv[3 * i + 1] - nz.mean[1]) / nz.std[1] : 0.0f;
+        f[6 * i + 2] = a ? (sv[3 * i + 2] - nz.mean[2]) / nz.std[2] : 0.0f;
+        f[6 * i + 3] = rain;
+        f[6 * i + 4] = ts[i];
+        f[6 * i + 5] = rs[i];
+      }
+      float4* t = reinterpret_cast<float4*>(tile) + threadIdx.x * 6;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) t[j] = make_float4(f[4 * j], f[4 * j + 1], f[4 * j + 2], f[4 * j + 3]);
+    }
+    __syncthreads();
+    const int nf4 = 6 * (int)min<size_t>(kBlock, N4 - base);  // float4s of this tile
+    float4* dst = o4 + base * 6;
+    for (int j = threadIdx.x; j < nf4; j += kBlock) dst[j] = reinterpret_cast<const float4*>(tile)[j];
+    __syncthreads();
+  }
+}
+
 // max(depth + residual[..., 0], 0) for a batch (v2_staging.py:47-52); residual
 // channel 0 of (s, k) is residual[s*ss + k*ps].
 __global__ __launch_bounds__(kBlock) void k_correct_depth(size_t N, const float* __restrict__ depth,
@@ -509,10 +562,21 @@ int swe_stage_features(int32_t count, int32_t ny, int32_t nx, const float* state
     nz.mean[i] = state_mean[i];
     nz.std[i] = state_std[i];
   }
-  const int nb = std::min(grid_blocks(N), 1024);
-  hipLaunchKernelGGL(k_stage, dim3(nb, count), dim3(kBlock), 0, (hipStream_t)stream, N, state_nhwc,
-                     domain_mask, terrain_std, roughness_std, nz, rain_channel, features,
-                     (long long)sample_stride, (long long)channel_stride, (long long)cell_stride);
+  auto al = [](const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; };
+  if (channel_stride == 1 && cell_stride == 6 && N % 4 == 0 && sample_stride % 4 == 0 &&
+      al(features, 16) && al(state_nhwc, 16) && al(domain_mask, 4) && al(terrain_std, 16) &&
+      al(roughness_std, 16)) {
+    const int nb = std::min(grid_blocks(N / 4), 1024);
+    hipLaunchKernelGGL(k_stage4, dim3(nb, count), dim3(kBlock), 0, (hipStream_t)stream, N,
+                       state_nhwc, domain_mask, terrain_std, roughness_std, nz, rain_channel,
+                       features, (long long)sample_stride);
+  } else {
+    const int nb = std::min(grid_blocks(N), 1024);
+    hipLaunchKernelGGL(k_stage, dim3(nb, count), dim3(kBlock), 0, (hipStream_t)stream, N,
+                       state_nhwc, domain_mask, terrain_std, roughness_std, nz, rain_channel,
+                       features, (long long)sample_stride, (long long)channel_stride,
+                       (long long)cell_stride);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(SWE_EHIP, std::string("k_stage: ") + hipGetErrorString(e));
   return SWE_OK;

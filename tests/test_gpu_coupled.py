@@ -292,14 +292,15 @@ def test_conserved_to_dataset_vector_and_scalar_forms(cuda, shape, offset):
 # Feature staging (comparison/v2_staging.py)
 # ---------------------------------------------------------------------------
 
-def test_stage_features_and_correct_depth_bit_exact(cuda):
+@pytest.mark.parametrize("ny,nx", [(19, 45), (20, 44)])  # one-cell; four-cell NHWC form
+def test_stage_features_and_correct_depth_bit_exact(cuda, ny, nx):
     import torch
 
     from hybrid_flood_amd.staging import correct_depth, stage_features
     from oracle import coupled_oracle as co
 
     rng = np.random.default_rng(9)
-    count, ny, nx = 4, 19, 45
+    count = 4
     st = rng.normal(0, 1, (count, ny, nx, 3)).astype(np.float32)
     mask = rng.uniform(size=(ny, nx)) > 0.2
     terrain, rough = rng.normal(size=(2, ny, nx)).astype(np.float32)
