@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/swe_hip.h"
@@ -451,6 +452,90 @@ int grid_blocks(size_t N) {
 
 bool valid(const SweState& s) { return s.h && s.hu && s.hv; }
 
+// apply_residual_correction on `scratch` (device: 4 result doubles, then 2 x nb
+// partials).  Shared by the entry point that keeps its scratch in the
+// context and the one that uses the file-scope array.
+int correct_with(const Grid& G, SweState predicted, const float* residual, int64_t channel_stride,
+                 int64_t cell_stride, float relaxation, const float* clip_limits,
+                 SweState corrected, double* volumes, double* scratch, int nb, hipStream_t s) {
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  Residual R;
+  R.r = residual;
+  R.cs = channel_stride;
+  R.ps = cell_stride;
+  R.relax = relaxation;
+  for (int i = 0; i < 3; ++i) R.lim[i] = clip_limits ? clip_limits[i] : HUGE_VALF;
+  double* part = scratch + 4;
+  hipLaunchKernelGGL(k_corr_sums, dim3(nb), dim3(kBlock), 0, s, G, predicted.h, R, part);
+  hipLaunchKernelGGL(k_corr_scale, dim3(1), dim3(kBlock), 0, s, part, nb, scratch);
+  // four-cell apply: N % 4 == 0, 16-byte arrays, a 4-byte mask, and an
+  // interleaved (3, 1) or planar (1, 4k) residual
+  auto al = [](const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; };
+  const bool interleaved = channel_stride == 1 && cell_stride == 3;
+  const bool planar = cell_stride == 1 && channel_stride % 4 == 0;
+  const bool vec = N % 4 == 0 && (interleaved || planar) && al(residual, 16) && al(G.mask, 4) &&
+                   al(predicted.h, 16) && al(predicted.hu, 16) && al(predicted.hv, 16) &&
+                   al(corrected.h, 16) && al(corrected.hu, 16) && al(corrected.hv, 16);
+  if (vec && interleaved)
+    hipLaunchKernelGGL(k_corr_apply4<true>, dim3(grid_blocks(N / 4)), dim3(kBlock), 0, s, G,
+                       predicted.h, predicted.hu, predicted.hv, R, scratch, corrected.h,
+                       corrected.hu, corrected.hv);
+  else if (vec)
+    hipLaunchKernelGGL(k_corr_apply4<false>, dim3(grid_blocks(N / 4)), dim3(kBlock), 0, s, G,
+                       predicted.h, predicted.hu, predicted.hv, R, scratch, corrected.h,
+                       corrected.hu, corrected.hv);
+  else
+    hipLaunchKernelGGL(k_corr_apply, dim3(grid_blocks(N)), dim3(kBlock), 0, s, G, predicted.h,
+                       predicted.hu, predicted.hv, R, scratch, corrected.h, corrected.hu,
+                       corrected.hv);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_error(SWE_EHIP, std::string("residual correction: ") + hipGetErrorString(e));
+  if (volumes) {
+    e = hipMemcpyAsync(volumes, scratch, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+      return set_error(SWE_EHIP, std::string("residual correction volumes: ") + hipGetErrorString(e));
+  }
+  return SWE_OK;
+}
+
+}  // namespace
+
+// Scratch of swe_apply_residual_correction_noalloc: 4 result doubles and the
+// 2 x kNoAllocParts per-workgroup volume partials, in device memory that the
+// code object itself holds (each device that loads the library keeps its own
+// copy, at its own address, until the process ends).  In host code the name
+// `scratch` is only a stand-in for the symbol: kernels get the device address
+// from hipGetSymbolAddress, kept per device in g_scratch_dev.
+constexpr int kNoAllocParts = 2048;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[4 + 2 * kNoAllocParts];
+namespace {
+double* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+}  // namespace
+
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return set_error(SWE_ENODEV, "noalloc scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return SWE_OK;
+  double* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return set_error(SWE_EHIP, std::string("noalloc scratch: hipGetSymbolAddress: ") +
+                                   hipGetErrorString(e));
+  g_scratch_dev[device] = ptr;
+  return SWE_OK;
+}
+
+namespace {
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_dev[device];
+}
 }  // namespace
 }  // namespace swe
 
@@ -503,46 +588,32 @@ int swe_apply_residual_correction(SweContext* c, SweState predicted, const float
   const int nb = std::min(grid_blocks(N), context_reduce_blocks(c));
   double* scratch = context_scratch(c, 2 * (size_t)nb + 4);
   if (!scratch) return set_error(SWE_EHIP, "swe_apply_residual_correction: scratch allocation failed");
-  Residual R;
-  R.r = residual;
-  R.cs = channel_stride;
-  R.ps = cell_stride;
-  R.relax = relaxation;
-  for (int i = 0; i < 3; ++i) R.lim[i] = clip_limits ? clip_limits[i] : HUGE_VALF;
-  hipStream_t s = (hipStream_t)stream;
-  double* part = scratch + 4;
-  hipLaunchKernelGGL(k_corr_sums, dim3(nb), dim3(kBlock), 0, s, G, predicted.h, R, part);
-  hipLaunchKernelGGL(k_corr_scale, dim3(1), dim3(kBlock), 0, s, part, nb, scratch);
-  // four-cell apply: N % 4 == 0, 16-byte arrays, a 4-byte mask, and an
-  // interleaved (3, 1) or planar (1, 4k) residual
-  auto al = [](const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; };
-  const bool interleaved = channel_stride == 1 && cell_stride == 3;
-  const bool planar = cell_stride == 1 && channel_stride % 4 == 0;
-  const bool vec = N % 4 == 0 && (interleaved || planar) && al(residual, 16) && al(G.mask, 4) &&
-                   al(predicted.h, 16) && al(predicted.hu, 16) && al(predicted.hv, 16) &&
-                   al(corrected.h, 16) && al(corrected.hu, 16) && al(corrected.hv, 16);
-  if (vec && interleaved)
-    hipLaunchKernelGGL(k_corr_apply4<true>, dim3(grid_blocks(N / 4)), dim3(kBlock), 0, s, G,
-                       predicted.h, predicted.hu, predicted.hv, R, scratch, corrected.h,
-                       corrected.hu, corrected.hv);
-  else if (vec)
-    hipLaunchKernelGGL(k_corr_apply4<false>, dim3(grid_blocks(N / 4)), dim3(kBlock), 0, s, G,
-                       predicted.h, predicted.hu, predicted.hv, R, scratch, corrected.h,
-                       corrected.hu, corrected.hv);
-  else
-    hipLaunchKernelGGL(k_corr_apply, dim3(grid_blocks(N)), dim3(kBlock), 0, s, G, predicted.h,
-                       predicted.hu, predicted.hv, R, scratch, corrected.h, corrected.hu,
-                       corrected.hv);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess)
-    return set_error(SWE_EHIP, std::string("residual correction: ") + hipGetErrorString(e));
-  if (volumes) {
-    e = hipMemcpyAsync(volumes, scratch, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess)
-      return set_error(SWE_EHIP, std::string("residual correction volumes: ") + hipGetErrorString(e));
-  }
-  return SWE_OK;
+  return correct_with(G, predicted, residual, channel_stride, cell_stride, relaxation, clip_limits,
+                      corrected, volumes, scratch, nb, (hipStream_t)stream);
+}
+
+int swe_apply_residual_correction_noalloc(SweContext* c, SweState predicted,
+                                          const float* residual, int64_t channel_stride,
+                                          int64_t cell_stride, float relaxation,
+                                          const float* clip_limits, SweState corrected,
+                                          double* volumes, void* stream) {
+  if (!c || !valid(predicted) || !valid(corrected) || !residual)
+    return set_error(SWE_EINVAL, "swe_apply_residual_correction_noalloc: null argument");
+  const Grid& G = *context_grid(c);
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  // The same workgroup count as swe_apply_residual_correction, so both sum
+  // the partials in the same order and give the same bits.
+  const int nb = std::min(grid_blocks(N), context_reduce_blocks(c));
+  if (nb > kNoAllocParts)
+    return set_error(SWE_EINVAL, "swe_apply_residual_correction_noalloc: the grid needs " +
+                                     std::to_string(nb) + " volume partials, more than the " +
+                                     std::to_string(kNoAllocParts) + " the static scratch holds");
+  double* ptr = noalloc_scratch(context_device(c));
+  if (!ptr)
+    return set_error(SWE_EHIP, "swe_apply_residual_correction_noalloc: the static scratch of "
+                               "this device was not resolved (swe_create does it)");
+  return correct_with(G, predicted, residual, channel_stride, cell_stride, relaxation, clip_limits,
+                      corrected, volumes, ptr, nb, (hipStream_t)stream);
 }
 
 int swe_stage_features(int32_t count, int32_t ny, int32_t nx, const float* state_nhwc,

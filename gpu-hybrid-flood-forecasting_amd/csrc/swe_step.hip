@@ -697,6 +697,7 @@ int swe::context_variant(const SweContext* c) { return c->variant; }
 double swe::context_gravity(const SweContext* c) { return c->gravity; }
 double swe::context_rate(const SweContext* c, double t) { return host_interp(c, t); }
 int swe::context_reduce_blocks(const SweContext* c) { return c->reduce_blocks(); }
+int swe::context_device(const SweContext* c) { return c->device; }
 int swe::context_adjoint_rows(const SweContext* c) {
   return c->adjoint_rows > 0 ? c->adjoint_rows : default_rows(c->G.nx, c->G.ny);
 }
@@ -815,6 +816,9 @@ int swe_create(const SweParamsDesc* p, int32_t variant, SweContext** out) {
   c->rows = default_rows(G.nx, G.ny);
   int rc = ensure_partials(c);
   if (rc) return cleanup(rc);
+  // Static scratch of the allocation-free residual correction, resolved once
+  // per device while this device is current.
+  if ((rc = noalloc_scratch_init(dev))) return cleanup(rc);
   // Sum of the rain multiplier over active cells, for the FP64 rain volume
   // (refreshed at every loop start).
   if (hipMalloc(&c->d_sum_mult, sizeof(double)) != hipSuccess)

@@ -352,6 +352,26 @@ int swe_apply_residual_correction(SweContext* ctx, SweState predicted, const flo
                                   const float* clip_limits, SweState corrected, double* volumes,
                                   void* stream);
 
+/* swe_apply_residual_correction without any device allocation, for callers
+ * that cannot allocate: code under HIP graph capture, tight serving loops
+ * that alternate the CNN and the physics, harnesses that forbid hipMalloc.
+ * No counterpart in the reference (jax allocates inside its jitted call).
+ * Same arguments, kernels and bits as swe_apply_residual_correction; its
+ * float64 volume partials live in a static device array of 4 + 2 x 2048
+ * doubles (32.8 KB, held by every device the library runs on until the
+ * process ends) instead of the context's lazily allocated scratch.  The
+ * array is resolved per device by swe_create.  Returns SWE_EINVAL if the
+ * grid would need more than 2048 partials (it never does: the partial count
+ * is capped at 2048 for any grid).  Every call on one device shares that
+ * array, so calls that overlap on one device (two streams) race; calls in
+ * one stream are ordered.  With `volumes` non-NULL the call synchronises the
+ * stream (do not capture it then); otherwise it is asynchronous. */
+int swe_apply_residual_correction_noalloc(SweContext* ctx, SweState predicted,
+                                          const float* residual, int64_t channel_stride,
+                                          int64_t cell_stride, float relaxation,
+                                          const float* clip_limits, SweState corrected,
+                                          double* volumes, void* stream);
+
 /* Feature staging of `count` primitive states (comparison/v2_staging.py:17-44):
  * state_nhwc is device float32 [count][ny][nx][3] (depth, u, v); output
  * channels (standardised state, masked; rain_channel[s] (device float32
