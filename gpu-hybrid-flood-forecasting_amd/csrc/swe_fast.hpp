@@ -219,12 +219,19 @@ __device__ __forceinline__ StripIO strip_io(const Grid& G, const float* H, const
 // one 32-bit load standing for all three elements; the whole-vector form is
 // split back into 32-bit loads; the volatile form is made system-coherent,
 // which bypasses the L2 that neighbouring strips share.)
+// With a rain multiplier that has one value on every active cell (UNI: the
+// reference's default multiplier, the domain mask, and any scalar multiple of
+// it) the static record drops it, 8 bytes per cell (bed, n'), and the step
+// takes the value as a constant: the rain source is the same fused multiply-add
+// with the same operands, so the bits do not change.
 struct PackedIO {
   const float* st;  // packed state
   const float* sx;  // packed static fields
   float* out;       // packed output state
-  unsigned off12;   // this lane's column, in bytes, in a packed row
-  size_t row3;      // floats per packed row (3 nx)
+  unsigned off12;   // this lane's column, in bytes, in a packed state row
+  unsigned offsx;   // this lane's column, in bytes, in a packed static row
+  size_t row3;      // floats per packed state row (3 nx)
+  size_t rowsx;     // floats per packed static row (3 nx, or 2 nx with UNI)
 };
 
 // A wave-uniform pointer, read from lane 0 so that it sits in scalar
@@ -238,13 +245,16 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
   return (T*)(((unsigned long long)hi << 32) | lo);
 }
 
+template <bool UNI>
 __device__ __forceinline__ PackedIO packed_io(const Grid& G, const float* S, float* O, int ic) {
   PackedIO io;
   io.st = uniform_ptr(S);
   io.sx = uniform_ptr(G.pk);
   io.out = uniform_ptr(O);
   io.off12 = (unsigned)ic * 12u;
+  io.offsx = (unsigned)ic * (UNI ? 8u : 12u);
   io.row3 = 3 * (size_t)G.nx;
+  io.rowsx = (UNI ? 2 : 3) * (size_t)G.nx;
   return io;
 }
 
@@ -265,24 +275,27 @@ __device__ __forceinline__ const GlobalF* packed_at(const float* base, size_t ro
 }
 
 // Row j of the lane's column: clamped row index (numerics.py:29-55); the
-// reflective-edge sign flips are applied in side_of.
-__device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, int j) {
+// reflective-edge sign flips are applied in side_of.  mc: the uniform
+// multiplier (UNI only).
+template <bool UNI>
+__device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, int j, float mc) {
   const int jj = min(max(j, 0), G.ny - 1);
   const GlobalF* st = packed_at(io.st, io.row3, jj, io.off12);
-  const GlobalF* sx = packed_at(io.sx, io.row3, jj, io.off12);
+  const GlobalF* sx = packed_at(io.sx, io.rowsx, jj, io.offsx);
   RawCell r;
   r.h = st[0];
   r.hu = st[1];
   r.hv = st[2];
   r.b = sx[0];
   r.n = sx[1];
-  r.m = sx[2];
+  r.m = UNI ? mc : sx[2];
   r.a = !signbit(r.n);  // sign bit clear: active
   r.neg_hv = (j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0);
   return r;
 }
 
-__device__ __forceinline__ RawCell load_raw(const Grid& G, const StripIO& io, int j) {
+template <bool UNI>
+__device__ __forceinline__ RawCell load_raw(const Grid& G, const StripIO& io, int j, float) {
   const int jj = min(max(j, 0), G.ny - 1);
   RawCell r;
   r.h = io.ld(io.h, jj);
@@ -317,7 +330,7 @@ __device__ __forceinline__ RowCell side_of(const RawCell& r, bool neg_hu, float 
 // row j+2 are in flight while row j is computed.  PACKED: H and Ho are packed
 // states (3 floats per cell, HU/HV unused) and the static fields come from
 // G.pk; the arithmetic is the same, so both layouts give identical bits.
-template <bool LOOP, bool PACKED, bool SOUTH>
+template <bool LOOP, bool PACKED, bool SOUTH, bool UNI>
 __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
                                                  const float* __restrict__ HV,
@@ -327,7 +340,8 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
                                                  double dyd, int rec_lo, int rec_hi,
                                                  float* __restrict__ cube_h,
                                                  float* __restrict__ cube_hu,
-                                                 float* __restrict__ cube_hv, StripAcc& acc) {
+                                                 float* __restrict__ cube_hv, StripAcc& acc,
+                                                 float mc) {
   const size_t N = (size_t)G.nx * (size_t)G.ny;
   const int lane = threadIdx.x & (kWave - 1);
   const int i = sx * kStripCols - 1 + lane;
@@ -342,7 +356,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   using IO = std::conditional_t<PACKED, PackedIO, StripIO>;
   IO io;
   if constexpr (PACKED)
-    io = packed_io(G, H, Ho, ic);
+    io = packed_io<UNI>(G, H, Ho, ic);
   else
     io = strip_io(G, H, HU, HV, max(j0 - 1, 0), ic);
   const size_t out0 = (size_t)j0 * (size_t)G.nx, out_cells = (size_t)(j1 - j0) * (size_t)G.nx;
@@ -359,8 +373,8 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   const int jf = SOUTH ? j1 - 1 : j0;  // first row of the march
   // Prologue: the face behind row jf, rows jf (current), jf+dj and jf+2dj
   // (in flight).
-  RowCell cur = side_of(load_raw(G, io, jf - dj), neg_hu, dry, G.half_g);
-  RawCell next = load_raw(G, io, jf);
+  RowCell cur = side_of(load_raw<UNI>(G, io, jf - dj, mc), neg_hu, dry, G.half_g);
+  RawCell next = load_raw<UNI>(G, io, jf, mc);
   RawCell r2;
   float yc0, yc1, yc2;  // carried y face: between the current row and the one behind it
   float nn, mult;
@@ -370,8 +384,8 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
       nn = next.n;
       mult = next.m;
     }
-    next = load_raw(G, io, jf + dj);
-    r2 = load_raw(G, io, SOUTH ? max(jf - 2, j0 - 1) : min(jf + 2, j1));
+    next = load_raw<UNI>(G, io, jf + dj, mc);
+    r2 = load_raw<UNI>(G, io, SOUTH ? max(jf - 2, j0 - 1) : min(jf + 2, j1), mc);
     if constexpr (!SOUTH) {  // south face of row j0
       const FaceFlux fy = face_fast_any<false>(G, cur.s, c0.s, wave_all(cur.s.a && c0.s.a));
       if (LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
@@ -402,7 +416,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
                  float nn_j, float mult_j, float& nn_next, float& mult_next) {
     // Unconditional, so the wait counts stay exact; past the strip the
     // clamped rows were just loaded and come from the cache.
-    ahead = load_raw(G, io, SOUTH ? max(j - 3, j0 - 1) : min(j + 3, j1));
+    ahead = load_raw<UNI>(G, io, SOUTH ? max(j - 3, j0 - 1) : min(j + 3, j1), mc);
     if constexpr (PACKED) {  // row j+dj's fields arrived with its state
       nn_next = north.n;
       mult_next = north.m;
@@ -569,7 +583,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
 // the second read is an L2 hit.  The direction does not change any cell's
 // arithmetic: every face takes the same operands, and a cell combines its
 // faces in the same order (north minus south, east minus west).
-template <bool LOOP, bool PACKED = false>
+template <bool LOOP, bool PACKED = false, bool UNI = false>
 __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
                                                  const float* __restrict__ HV,
@@ -579,13 +593,17 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
                                                  double dyd, int rec_lo, int rec_hi,
                                                  float* __restrict__ cube_h,
                                                  float* __restrict__ cube_hu,
-                                                 float* __restrict__ cube_hv, StripAcc& acc) {
+                                                 float* __restrict__ cube_hv, StripAcc& acc,
+                                                 float mc = 0.0f) {
+  static_assert(PACKED || !UNI, "the uniform-multiplier record is a packed layout");
   if (sy & 1)
-    march_strip_dir<LOOP, PACKED, true>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dt, dtrate, dtg,
-                                        dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
+    march_strip_dir<LOOP, PACKED, true, UNI>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dt, dtrate,
+                                             dtg, dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu,
+                                             cube_hv, acc, mc);
   else
-    march_strip_dir<LOOP, PACKED, false>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dt, dtrate, dtg,
-                                         dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
+    march_strip_dir<LOOP, PACKED, false, UNI>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dt, dtrate,
+                                              dtg, dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu,
+                                              cube_hv, acc, mc);
 }
 
 }  // namespace swe

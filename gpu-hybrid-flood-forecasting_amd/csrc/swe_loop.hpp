@@ -43,7 +43,9 @@ struct LoopControl {
 struct LoopConst {
   double cfl, max_dt, gravity, cell_area, rain_default, dxd, dyd;
   // FP64 sum of the rain multiplier over active cells (device-resident, so a
-  // loop start can refresh it without a host round trip).
+  // loop start can refresh it without a host round trip).  The single-domain
+  // context also keeps [1] = 1 if the multiplier has one value on every
+  // active cell and [2] = that value, read by the packed device loop.
   const double* sum_mult;
   const double* rain_t;
   const double* rain_r;

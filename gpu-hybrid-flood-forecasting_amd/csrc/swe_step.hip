@@ -73,10 +73,21 @@ __global__ __launch_bounds__(kBlock) void k_step(
   if (rev && gw < n_waves) gw = n_waves - 1 - gw;
   StripAcc acc;
   if (gw < n_waves) {
-    if constexpr (M::kFast)
-      march_strip_fast<MODE != MODE_STEP, PACKED>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
-                                                  gw / segs_x, rows, dt, dtrate, dtg, L.dxd, L.dyd,
-                                                  rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
+    if constexpr (M::kFast && PACKED) {
+      // L.sum_mult[1]: the rain multiplier has one value on every active
+      // cell, L.sum_mult[2]; the packed static record then leaves it out.
+      if (L.sum_mult[1] != 0.0)
+        march_strip_fast<true, true, true>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
+                                           rows, dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
+                                           cube_h, cube_hu, cube_hv, acc, (float)L.sum_mult[2]);
+      else
+        march_strip_fast<true, true, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
+                                            rows, dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
+                                            cube_h, cube_hu, cube_hv, acc);
+    } else if constexpr (M::kFast)
+      march_strip_fast<MODE != MODE_STEP, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
+                                                 gw / segs_x, rows, dt, dtrate, dtg, L.dxd, L.dyd,
+                                                 rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
     else
       march_strip<M, MODE != MODE_STEP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
                                         rows, dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
@@ -181,24 +192,67 @@ __global__ __launch_bounds__(kBlock) void k_cfl_field(Grid G, const float* __res
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_sum_mult(Grid G, double* __restrict__ partial) {
+// Per workgroup: the FP64 sum of the rain multiplier over active cells and
+// the smallest and largest bit pattern it takes there (equal bit patterns on
+// every active cell make the multiplier uniform).
+__global__ __launch_bounds__(kBlock) void k_sum_mult(Grid G, double* __restrict__ sum_part,
+                                                     unsigned* __restrict__ bits_part) {
   const size_t N = (size_t)G.nx * (size_t)G.ny;
   double s = 0.0;
+  unsigned lo = 0xFFFFFFFFu, hi = 0u;
   for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
        k += (size_t)gridDim.x * kBlock) {
-    if (G.mask[k]) s += (double)G.mult[k];
+    if (G.mask[k]) {
+      const float m = G.mult[k];
+      const unsigned b = __float_as_uint(m);
+      s += (double)m;
+      lo = min(lo, b);
+      hi = max(hi, b);
+    }
   }
-  const Totals t = block_reduce(0.0f, s, 0.0, 0.0, 0);
-  if (threadIdx.x == 0) partial[blockIdx.x] = t.vol;
+  __shared__ unsigned s_lo[kBlock], s_hi[kBlock];
+  s_lo[threadIdx.x] = lo;
+  s_hi[threadIdx.x] = hi;
+  const Totals t = block_reduce(0.0f, s, 0.0, 0.0, 0);  // synchronises
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock; ++k) {
+      lo = min(lo, s_lo[k]);
+      hi = max(hi, s_hi[k]);
+    }
+    sum_part[blockIdx.x] = t.vol;
+    bits_part[2 * blockIdx.x] = lo;
+    bits_part[2 * blockIdx.x + 1] = hi;
+  }
 }
 
-// One workgroup: out = the sum of partial[0 .. n) in a fixed order.
-__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partial, int n,
-                                                         double* __restrict__ out) {
+// One workgroup: out[0] = the sum of sum_part[0 .. n) in a fixed order;
+// out[1] = 1 if the multiplier has one bit pattern on every active cell (or
+// there is no active cell), else 0; out[2] = that value (0 without active
+// cells).
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ sum_part,
+                                                         const unsigned* __restrict__ bits_part,
+                                                         int n, double* __restrict__ out) {
   double s = 0.0;
-  for (int k = threadIdx.x; k < n; k += kBlock) s += partial[k];
-  const Totals t = block_reduce(0.0f, s, 0.0, 0.0, 0);
-  if (threadIdx.x == 0) *out = t.vol;
+  unsigned lo = 0xFFFFFFFFu, hi = 0u;
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    s += sum_part[k];
+    lo = min(lo, bits_part[2 * k]);
+    hi = max(hi, bits_part[2 * k + 1]);
+  }
+  __shared__ unsigned s_lo[kBlock], s_hi[kBlock];
+  s_lo[threadIdx.x] = lo;
+  s_hi[threadIdx.x] = hi;
+  const Totals t = block_reduce(0.0f, s, 0.0, 0.0, 0);  // synchronises
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock; ++k) {
+      lo = min(lo, s_lo[k]);
+      hi = max(hi, s_hi[k]);
+    }
+    const bool none = lo > hi;  // no active cell
+    out[0] = t.vol;
+    out[1] = (none || lo == hi) ? 1.0 : 0.0;
+    out[2] = none ? 0.0 : (double)__uint_as_float(lo);
+  }
 }
 
 // Copy the loop's final state (buffer A or B by the parity of the step count)
@@ -214,13 +268,23 @@ __global__ __launch_bounds__(kBlock) void k_fill3(float* __restrict__ a, float* 
 }
 
 // Packed layout of the fast device loop (see swe_fast.hpp): static fields and
-// state in, state out.
-__global__ __launch_bounds__(kBlock) void k_pack_static(Grid G, float* __restrict__ pk, size_t N) {
+// state in, state out.  The static record is (bed, n') when the multiplier is
+// uniform over the active cells (sm[1], from k_sum_partials), else (bed, n',
+// multiplier).
+__global__ __launch_bounds__(kBlock) void k_pack_static(Grid G, float* __restrict__ pk, size_t N,
+                                                        const double* __restrict__ sm) {
+  const bool uni = sm[1] != 0.0;
   for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
        k += (size_t)gridDim.x * kBlock) {
-    pk[3 * k] = G.bed[k];
-    pk[3 * k + 1] = G.mask[k] ? G.n[k] * G.n[k] : -1.0f;
-    pk[3 * k + 2] = G.mult[k];
+    const float b = G.bed[k], n2 = G.mask[k] ? G.n[k] * G.n[k] : -1.0f;
+    if (uni) {
+      pk[2 * k] = b;
+      pk[2 * k + 1] = n2;
+    } else {
+      pk[3 * k] = b;
+      pk[3 * k + 1] = n2;
+      pk[3 * k + 2] = G.mult[k];
+    }
   }
 }
 
@@ -304,7 +368,9 @@ struct SweContext {
   int graph_chunk = 64;
   double dx = 1, dy = 1, gravity = 9.81, cfl = 0.45, dry = 1e-4, max_dt = 10, rain_default = 0;
   std::vector<double> rain_t, rain_r;
-  double* d_sum_mult = nullptr;  // FP64 rain-multiplier sum over active cells
+  // [FP64 rain-multiplier sum over active cells, 1 if the multiplier is
+  // uniform there, its value, unused]
+  double* d_sum_mult = nullptr;
   double* d_rain = nullptr;  // [rain_t | rain_r]
   double* d_times = nullptr;
   int times_cap = 0;
@@ -428,8 +494,11 @@ int ensure_work(SweContext* c) {
 // partials as scratch.
 int refresh_sum_mult(SweContext* c, hipStream_t s) {
   const int nb = c->reduce_blocks();
-  hipLaunchKernelGGL(k_sum_mult, dim3(nb), dim3(kBlock), 0, s, c->G, c->P.vol);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, c->P.vol, nb, c->d_sum_mult);
+  // The partials' clip array (nb doubles) holds the 2 nb bit-pattern bounds.
+  unsigned* bits = reinterpret_cast<unsigned*>(c->P.clip);
+  hipLaunchKernelGGL(k_sum_mult, dim3(nb), dim3(kBlock), 0, s, c->G, c->P.vol, bits);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, c->P.vol, bits, nb,
+                     c->d_sum_mult);
   SWE_HIP_TRY(hipGetLastError());
   return SWE_OK;
 }
@@ -440,7 +509,7 @@ int stage_in(SweContext* c, const SweState& src, hipStream_t s) {
   if (!c->loop_packed) return copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), src.h, src.hu, src.hv, s);
   const size_t N = c->cells();
   const int nb = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 8192);
-  hipLaunchKernelGGL(k_pack_static, dim3(nb), dim3(kBlock), 0, s, c->G, c->pk, N);
+  hipLaunchKernelGGL(k_pack_static, dim3(nb), dim3(kBlock), 0, s, c->G, c->pk, N, c->d_sum_mult);
   hipLaunchKernelGGL(k_pack_state, dim3(nb), dim3(kBlock), 0, s, src.h, src.hu, src.hv, c->ws(0), N);
   SWE_HIP_TRY(hipGetLastError());
   return SWE_OK;
@@ -821,7 +890,7 @@ int swe_create(const SweParamsDesc* p, int32_t variant, SweContext** out) {
   if ((rc = noalloc_scratch_init(dev))) return cleanup(rc);
   // Sum of the rain multiplier over active cells, for the FP64 rain volume
   // (refreshed at every loop start).
-  if (hipMalloc(&c->d_sum_mult, sizeof(double)) != hipSuccess)
+  if (hipMalloc(&c->d_sum_mult, 4 * sizeof(double)) != hipSuccess)
     return cleanup(fail(SWE_EHIP, "swe_create: hipMalloc rain multiplier sum"));
   if ((rc = refresh_sum_mult(c, nullptr))) return cleanup(rc);
   if (hipDeviceSynchronize() != hipSuccess)

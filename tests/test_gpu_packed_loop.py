@@ -37,15 +37,31 @@ def _state(case, dev):
     return nm.SWEState(*(torch.as_tensor(case[k], device=dev).contiguous() for k in ("h", "hu", "hv")))
 
 
+def _set_multiplier(case, form, rng):
+    """field: per-cell values (the 3-float static record); uniform: 1.3 on
+    every active cell and junk on inactive ones (the 2-float record with the
+    value as a constant; inactive cells never use it); mask: the reference's
+    default, the mask as float32."""
+    mask = case["domain_mask"]
+    if form == "uniform":
+        junk = rng.uniform(-5.0, 5.0, mask.shape)
+        case["rainfall_multiplier"] = np.where(mask, 1.3, junk).astype(np.float32)
+    elif form == "mask":
+        case["rainfall_multiplier"] = None
+    return case
+
+
+@pytest.mark.parametrize("mult", ["field", "uniform", "mask"])
 @pytest.mark.parametrize("shape,sides", [((41, 77), (0, 1, 0, 0)), ((64, 130), (1, 1, 1, 1))])
-def test_loop_equals_step_replay_with_negative_n(cuda, shape, sides):
+def test_loop_equals_step_replay_with_negative_n(cuda, shape, sides, mult):
     from hybrid_flood_amd import numerics as nm
     from hybrid_flood_amd.adjoint import rollout_forward
 
-    case = random_case(17, shape, open_sides=sides, speed=0.8)
+    case = random_case(17, shape, open_sides=sides, speed=0.8, rain=2.0e-4)
     rng = np.random.default_rng(3)
     flip = rng.uniform(size=shape) < 0.3
     case["manning_n"] = np.where(flip, -case["manning_n"], case["manning_n"]).astype(np.float32)
+    case = _set_multiplier(case, mult, rng)
     params = _params(case, cuda)
     times = np.array([0.0, 2.0, 5.0], np.float32)
     _, res, tape = rollout_forward(params, _state(case, cuda), times, max_steps=3000, variant="fast",
@@ -85,6 +101,8 @@ def test_sweep_order_does_not_change_the_state(cuda, rows):
 
 
 def test_fields_changed_in_place_are_used(cuda):
+    import torch
+
     from hybrid_flood_amd import numerics as nm
 
     case = random_case(23, (48, 70), open_sides=(0, 1, 0, 0), speed=0.8)
@@ -109,6 +127,22 @@ def test_fields_changed_in_place_are_used(cuda):
     assert second.diagnostics["rain_volume"] != pytest.approx(first.diagnostics["rain_volume"],
                                                               rel=1e-3)
     assert abs(second.diagnostics["mass_balance_rel"]) < 1e-6
+    # a per-cell multiplier made uniform in place switches the packed static
+    # record to the 2-float form, and back
+    params.rainfall_multiplier.copy_(torch.where(params.domain_mask != 0, 0.8, -3.0))
+    third = nm.integrate_to_outputs(_state(case, cuda), params, times, max_steps=2000, variant="fast")
+    case3 = dict(case2, rainfall_multiplier=params.rainfall_multiplier.cpu().numpy())
+    fresh3 = nm.integrate_to_outputs(_state(case, cuda), _params(case3, cuda), times, max_steps=2000,
+                                     variant="fast")
+    for a, b in zip(third.states, fresh3.states):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    assert third.diagnostics["rain_volume"] == pytest.approx(fresh3.diagnostics["rain_volume"],
+                                                             rel=1e-12)
+    assert abs(third.diagnostics["mass_balance_rel"]) < 1e-6
+    params.rainfall_multiplier.copy_(torch.as_tensor(case2["rainfall_multiplier"], device=cuda))
+    fourth = nm.integrate_to_outputs(_state(case, cuda), params, times, max_steps=2000, variant="fast")
+    for a, b in zip(fourth.states, second.states):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
 
 
 def test_variant_switch_on_one_context(cuda):
