@@ -25,6 +25,22 @@ inline int default_rows(int nx, int ny) {
   return r < 4.0 ? 4 : (r > (double)kDefaultRows ? kDefaultRows : (int)r);
 }
 
+// Plan of one step of the fused device loop (MODE_LOOP_FUSED): what every
+// workgroup of the step reads on entry, besides the CFL maximum of the state,
+// to derive the step's dt, rain rate and output slots itself.  Written by
+// workgroup 0 of the step before (or by the loop start), double-buffered by
+// step parity.
+struct StepPlan {
+  double t;          // clock at the start of the step
+  double rate;       // rain rate at t (float64 interpolation)
+  double tw[4];      // times[oi], times[oi+1], ... (+inf past the last output time)
+  double prev_dt;    // dt of the step before (for its FP64 accounting)
+  float prev_dtrate; // float32(dt * rate) of the step before
+  int steps;         // steps taken before this one (-1: no such step)
+  int oi;            // first output slot not reached at t
+  int done;          // every output reached or max_steps taken: the loop stops here
+};
+
 // Device-resident state of the time loop.  The first fields are what every
 // step kernel reads on entry.
 struct LoopControl {
@@ -38,6 +54,9 @@ struct LoopControl {
   double max_inv;
   int nonfinite;
   unsigned counter;  // workgroups finished in the current launch
+  StepPlan plan[2];  // fused loop: plans of the steps of parity 0 and 1
+  int acc_steps;     // fused loop: steps whose FP64 accounting is in the fields above
+  int _pad;
 };
 
 struct LoopConst {
@@ -68,12 +87,26 @@ struct Partials {
   double* clip;
   double* bnd;
   int* nf;
+  // Fused loop: the partials of a step of parity p are at [p * stride, ...),
+  // and the CFL maximum of the state a step produces goes to the 64 slots
+  // cfl_slots[64 (1 - p) ...] as (steps after it) << 32 | float bits,
+  // 0xFFFFFFFF for a non-finite state.
+  unsigned long long* cfl_slots;
+  int stride;
 };
+
+constexpr int kCflSlots = 64;
+constexpr unsigned kNonfiniteBits = 0xFFFFFFFFu;
 
 // MODE_STEP: caller-given dt, no epilogue.  MODE_LOOP: device loop, the last
 // workgroup finishes the step.  MODE_LOOP_SPLIT: device loop, every
 // workgroup stores its partials and a one-workgroup kernel finishes the step.
-enum { MODE_STEP = 0, MODE_LOOP = 1, MODE_LOOP_SPLIT = 2 };
+// MODE_LOOP_FUSED: device loop without a finish kernel: each workgroup
+// folds its CFL maximum into the step's slots with an atomic max; every
+// workgroup of the next step reads them and derives dt itself, and its
+// workgroup 0 plans the step after and does the FP64 accounting of the one
+// before (k_close accounts the last step when the loop ends).
+enum { MODE_STEP = 0, MODE_LOOP = 1, MODE_LOOP_SPLIT = 2, MODE_LOOP_FUSED = 3 };
 enum { RED_CFL = 0, RED_INIT = 1 };
 
 template <class T>
@@ -198,17 +231,25 @@ __device__ int rain_bracket(const LoopConst& L, double t) {
   return s_j;
 }
 
+// FP64 accounting of one finished step (thread 0): rain, open-boundary and
+// clip volumes, the volume after it and the non-finite flag.  dt and dtrate
+// are the step's own (finish_step and the fused loop share this arithmetic).
+__device__ __forceinline__ void account_step(LoopControl* ctl, const LoopConst& L,
+                                             const Totals& tot, double dt, float dtrate) {
+  ctl->rain += (double)dtrate * *L.sum_mult * L.cell_area;
+  ctl->bnd += dt * tot.bnd;
+  ctl->clip += tot.clip * L.cell_area;
+  ctl->vol = tot.vol * L.cell_area;
+  ctl->nonfinite |= tot.nf;
+}
+
 // Close the step just taken (single thread): account its rain, open-boundary
 // and clip volumes in FP64, advance the clock and step count, move past the
 // output times it reached (numerics.py:398-412), then stop or plan the next
 // step.  Resets the workgroup counter for the next launch.
 __device__ inline void finish_step(LoopControl* ctl, const LoopConst& L, const Totals& tot,
                                    double t_new, int rain_j) {
-  ctl->rain += (double)ctl->dtrate_f * *L.sum_mult * L.cell_area;
-  ctl->bnd += ctl->dt * tot.bnd;
-  ctl->clip += tot.clip * L.cell_area;
-  ctl->vol = tot.vol * L.cell_area;
-  ctl->nonfinite |= tot.nf;
+  account_step(ctl, L, tot, ctl->dt, ctl->dtrate_f);
   ctl->max_inv = (double)tot.inv;
   const int steps = ctl->steps + 1;
   ctl->steps = steps;
@@ -224,6 +265,147 @@ __device__ inline void finish_step(LoopControl* ctl, const LoopConst& L, const T
     plan_next_step(ctl, L, tot.inv, t_new, oi, rain_j);
   }
   __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Block-wide totals of the partials [first, first + count) into thread 0.
+// Thread t adds entries t, t + BLOCK, t + 2 BLOCK, ... in that order whatever
+// UNROLL is (UNROLL only sets how many loads are in flight), so every caller
+// sums in the same order and gets the same bits.  INV: also the float CFL
+// maximum (the fused loop keeps it in cfl_slots instead).
+template <int BLOCK, int UNROLL, bool INV, class Parts>
+__device__ Totals reduce_partials(const Parts& P, int first, int count) {
+  const int tid = threadIdx.x;
+  float inv = 0.0f;
+  double vol = 0.0, clip = 0.0, bnd = 0.0;
+  int nf = 0;
+  for (int base = tid; base < count; base += BLOCK * UNROLL) {
+    float pi[UNROLL];
+    double pv[UNROLL], pc[UNROLL], pb[UNROLL];
+    int pn[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      const int k = base + u * BLOCK;
+      const bool in = k < count;
+      const int q = first + (in ? k : 0);
+      pi[u] = (INV && in) ? P.inv[q] : 0.0f;
+      pv[u] = in ? P.vol[q] : 0.0;
+      pc[u] = in ? P.clip[q] : 0.0;
+      pb[u] = in ? P.bnd[q] : 0.0;
+      pn[u] = in ? P.nf[q] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      inv = fmaxf(inv, pi[u]);
+      vol += pv[u];
+      clip += pc[u];
+      bnd += pb[u];
+      nf |= pn[u];
+    }
+  }
+  return block_reduce<BLOCK / kWave>(inv, vol, clip, bnd, nf);
+}
+
+// ---- fused device loop -------------------------------------------------------
+
+// The CFL maximum of the state entering step `steps`, as float bits
+// (kNonfiniteBits for a non-finite state), from the 64 slots that the step
+// before filled.  Slots of an older step (smaller tag) count as 0.  Called by
+// every lane of a wavefront; the result is wave-uniform.
+__device__ __forceinline__ unsigned cfl_bits(const unsigned long long* slots, int steps) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned long long w =
+      __hip_atomic_load(slots + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned b = ((unsigned)(w >> 32) == (unsigned)steps) ? (unsigned)w : 0u;
+  for (int o = kWave / 2; o > 0; o >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, o, kWave));
+  return __builtin_amdgcn_readfirstlane(b);
+}
+
+// What a workgroup of the fused loop derives on entry from the step's plan
+// and the CFL maximum m of its state: the same float64 arithmetic as
+// plan_next_step (numerics.py:405 with the float64 quotient of
+// numpy_solver/shallow_water_2d.py:274), so every workgroup gets the same dt.
+struct StepGo {
+  double dt;
+  float dt_f, dtrate_f, dtg_f, m;
+  int rec_lo, rec_hi, ck_slot;
+  bool stop, nonfinite;
+};
+
+__device__ __forceinline__ StepGo step_go(const StepPlan& pl, unsigned bits, const LoopConst& L) {
+  StepGo g;
+  g.nonfinite = bits == kNonfiniteBits;
+  g.stop = pl.done != 0 || g.nonfinite;
+  g.m = __uint_as_float(g.nonfinite ? 0u : bits);
+  g.dt = 0.0;
+  g.dt_f = g.dtrate_f = g.dtg_f = 0.0f;
+  g.rec_lo = g.rec_hi = 0;
+  g.ck_slot = -1;
+  if (g.stop) return g;
+  const double md = (double)g.m;
+  const double cfl_dt = md > 0.0 ? fmin(L.cfl / md, L.max_dt) : L.max_dt;
+  const double dt = fmin(cfl_dt, pl.tw[0] - pl.t);
+  const double t_end = pl.t + dt;
+  // Output times the step reaches: the plan's window of four, then (rare:
+  // more than four output times in one step) the series itself.
+  int hi = pl.oi;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (hi == pl.oi + k && hi < L.n_times && t_end >= pl.tw[k] - kOutputSlack) ++hi;
+  if (hi == pl.oi + 4)
+    while (hi < L.n_times && t_end >= L.times[hi] - kOutputSlack) ++hi;
+  g.rec_lo = pl.oi;
+  g.rec_hi = L.record ? hi : pl.oi;
+  const int after = pl.steps + 1;
+  const int slot = L.ck_every > 0 ? after / L.ck_every : 0;
+  g.ck_slot = (L.ck_every > 0 && after % L.ck_every == 0 && slot < L.ck_cap && g.rec_lo == g.rec_hi)
+                  ? slot
+                  : -1;
+  g.dt = dt;
+  g.dt_f = (float)dt;
+  g.dtrate_f = (float)(dt * pl.rate);
+  g.dtg_f = (float)(dt * L.gravity);
+  return g;
+}
+
+// The plan of the step after `pl` (workgroup 0 of the step, every thread):
+// clock, output slot and rain rate after the step, as finish_step derives
+// them; also the step's entry of the dt log.
+__device__ inline void write_next_plan(StepPlan* __restrict__ nx, const LoopConst& L,
+                                       const StepPlan& pl, const StepGo& g) {
+  const double t1 = pl.t + g.dt;
+  const int rj = rain_bracket(L, t1);
+  if (threadIdx.x == 0) {
+    int oi = pl.oi;
+    while (oi < L.n_times && t1 >= L.times[oi] - kOutputSlack) ++oi;
+    nx->t = t1;
+    nx->rate = interp_at(L.rain_t, L.rain_r, L.n_rain, L.rain_default, t1, rj);
+    for (int k = 0; k < 4; ++k) nx->tw[k] = oi + k < L.n_times ? L.times[oi + k] : HUGE_VAL;
+    nx->prev_dt = g.dt;
+    nx->prev_dtrate = g.dtrate_f;
+    nx->steps = pl.steps + 1;
+    nx->oi = oi;
+    nx->done = (oi >= L.n_times || pl.steps + 1 >= L.max_steps) ? 1 : 0;
+    if (L.dt_log && pl.steps < L.dt_log_cap) L.dt_log[pl.steps] = g.dt;
+  }
+}
+
+// Account the step before `pl` (its partials at [first, first + count)) and
+// record the loop's position at the start of `pl` (every thread of one
+// workgroup).  m: CFL maximum of the state entering `pl`.
+template <int UNROLL>
+__device__ inline void account_before(LoopControl* __restrict__ ctl, const LoopConst& L,
+                                      const Partials& P, int first, int count,
+                                      const StepPlan& pl, float m, bool nonfinite) {
+  const Totals tot = reduce_partials<kBlock, UNROLL, false>(P, first, count);
+  if (threadIdx.x == 0) {
+    account_step(ctl, L, tot, pl.prev_dt, pl.prev_dtrate);
+    ctl->nonfinite |= nonfinite ? 1 : 0;
+    ctl->max_inv = (double)m;
+    ctl->steps = pl.steps;
+    ctl->t = pl.t;
+    ctl->out_idx = pl.oi;
+    ctl->acc_steps = pl.steps;
+  }
 }
 
 // Series a finish kernel stages in LDS when they are this short or shorter.
@@ -253,34 +435,7 @@ __device__ void finish_block(LoopControl* __restrict__ ctl, const LoopConst& L, 
     }
   if (times_lds)
     for (int k = tid; k < L.n_times; k += BLOCK) s_tm[k] = L.times[k];
-  float inv = 0.0f;
-  double vol = 0.0, clip = 0.0, bnd = 0.0;
-  int nf = 0;
-  for (int base = tid; base < count; base += BLOCK * UNROLL) {
-    float pi[UNROLL];
-    double pv[UNROLL], pc[UNROLL], pb[UNROLL];
-    int pn[UNROLL];
-#pragma unroll
-    for (int u = 0; u < UNROLL; ++u) {
-      const int k = base + u * BLOCK;
-      const bool in = k < count;
-      const int q = first + (in ? k : 0);
-      pi[u] = in ? P.inv[q] : 0.0f;
-      pv[u] = in ? P.vol[q] : 0.0;
-      pc[u] = in ? P.clip[q] : 0.0;
-      pb[u] = in ? P.bnd[q] : 0.0;
-      pn[u] = in ? P.nf[q] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < UNROLL; ++u) {
-      inv = fmaxf(inv, pi[u]);
-      vol += pv[u];
-      clip += pc[u];
-      bnd += pb[u];
-      nf |= pn[u];
-    }
-  }
-  const Totals tot = block_reduce<BLOCK / kWave>(inv, vol, clip, bnd, nf);  // syncs the LDS copies
+  const Totals tot = reduce_partials<BLOCK, UNROLL, true>(P, first, count);  // syncs the LDS copies
   if (s_ctl.done) return;  // block-uniform
   LoopConst Ls = L;
   if (rain_lds) {

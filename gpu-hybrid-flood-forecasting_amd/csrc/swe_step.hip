@@ -37,16 +37,55 @@ namespace swe {
 // ---------------------------------------------------------------------------
 // The fused step
 // ---------------------------------------------------------------------------
+// Partials in flight per thread when workgroup 0 of a fused-loop step sums
+// the step before's (few: it shares the step kernel's register budget).
+constexpr int kFusedUnroll = 2;
+
 template <class M, int MODE, bool PACKED = false>
 __global__ __launch_bounds__(kBlock) void k_step(
     Grid G, const float* __restrict__ H, const float* __restrict__ HU,
     const float* __restrict__ HV, float* __restrict__ Ho, float* __restrict__ HUo,
     float* __restrict__ HVo, float dt_arg, float dtrate_arg, float dtg_arg, int rows,
-    int segs_x, int n_waves, int rev, LoopControl* __restrict__ ctl, LoopConst L, Partials P,
-    float* __restrict__ cube_h, float* __restrict__ cube_hu, float* __restrict__ cube_hv) {
+    int segs_x, int n_waves, int rev, int par, LoopControl* __restrict__ ctl, LoopConst L,
+    Partials P, float* __restrict__ cube_h, float* __restrict__ cube_hu,
+    float* __restrict__ cube_hv) {
   float dt = dt_arg, dtrate = dtrate_arg, dtg = dtg_arg;
   int rec_lo = 0, rec_hi = 0;
-  if constexpr (MODE != MODE_STEP) {
+  int steps_before = 0;  // fused loop: index of this step
+  if constexpr (MODE == MODE_LOOP_FUSED) {
+    // par: parity of this step.  Every workgroup derives the step from its
+    // plan and the CFL maximum that the previous step left in the slots.
+    if (ctl->done) return;
+    const StepPlan pl = ctl->plan[par];
+    const StepGo go = step_go(pl, cfl_bits(P.cfl_slots + kCflSlots * par, pl.steps), L);
+    steps_before = pl.steps;
+    if (go.stop) {
+      // Every workgroup sees the same plan and slots, so all stop here;
+      // workgroup 0 accounts the last step taken and marks the loop done.
+      if (blockIdx.x == 0) {
+        if (pl.steps >= 1 && ctl->acc_steps < pl.steps)
+          account_before<kFusedUnroll>(ctl, L, P, (1 - par) * P.stride, (int)gridDim.x, pl,
+                                       go.m, go.nonfinite);
+        if (threadIdx.x == 0) {
+          ctl->nonfinite |= go.nonfinite ? 1 : 0;
+          ctl->done = 1;
+        }
+      }
+      return;
+    }
+    dt = go.dt_f;
+    dtrate = go.dtrate_f;
+    dtg = go.dtg_f;
+    rec_lo = go.rec_lo;
+    rec_hi = go.rec_hi;
+    if (go.ck_slot >= 0) {
+      cube_h = L.ck_h;
+      cube_hu = L.ck_hu;
+      cube_hv = L.ck_hv;
+      rec_lo = go.ck_slot;
+      rec_hi = go.ck_slot + 1;
+    }
+  } else if constexpr (MODE != MODE_STEP) {
     if (ctl->done) return;
     dt = ctl->dt_f;
     dtrate = ctl->dtrate_f;
@@ -105,6 +144,35 @@ __global__ __launch_bounds__(kBlock) void k_step(
       P.nf[b] = mine.nf;
     }
   }
+  if constexpr (MODE == MODE_LOOP_FUSED) {
+    const Totals mine = block_reduce(acc.inv, acc.vol, acc.clip, acc.bnd, acc.nf);
+    if (threadIdx.x == 0) {
+      const int q = par * P.stride + (int)blockIdx.x;
+      P.vol[q] = mine.vol;
+      P.clip[q] = mine.clip;
+      P.bnd[q] = mine.bnd;
+      P.nf[q] = mine.nf;
+      const unsigned bits = mine.nf ? kNonfiniteBits : __float_as_uint(mine.inv);
+      const unsigned long long tag = (unsigned long long)(unsigned)(steps_before + 1) << 32;
+      __hip_atomic_fetch_max(P.cfl_slots + kCflSlots * (1 - par) + (blockIdx.x & (kCflSlots - 1)),
+                             tag | bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (blockIdx.x == 0) {
+      // Workgroup 0: the plan of the next step, and the FP64 accounting of
+      // the step before this one (its partials are complete: that kernel
+      // ended before this one started).  The plan is read and the step
+      // derived again rather than kept in registers through the march (the
+      // memory clobber forces the reload; nothing this kernel writes changes
+      // them).
+      asm volatile("" ::: "memory");
+      const StepPlan pl = ctl->plan[par];
+      const StepGo go = step_go(pl, cfl_bits(P.cfl_slots + kCflSlots * par, pl.steps), L);
+      write_next_plan(&ctl->plan[1 - par], L, pl, go);
+      if (pl.steps >= 1)
+        account_before<kFusedUnroll>(ctl, L, P, (1 - par) * P.stride, (int)gridDim.x, pl, go.m,
+                                     false);
+    }
+  }
   if constexpr (MODE == MODE_LOOP) {
     const Totals mine = block_reduce(acc.inv, acc.vol, acc.clip, acc.bnd, acc.nf);
     if (!publish_partials(P, ctl, mine)) return;
@@ -131,6 +199,22 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish(LoopControl* __restrict
   finish_block<kFinishBlock, kFinishUnroll>(ctl, L, P, 0, nparts);
 }
 
+// End of a fused device loop (one workgroup): account the last step taken if
+// no later step kernel did (the loop ran out of launches before it stopped
+// itself), and leave the clock, step count and output cursor after it.
+__global__ __launch_bounds__(kBlock) void k_close(LoopControl* __restrict__ ctl, LoopConst L,
+                                                  Partials P, int nparts) {
+  const int executed = max(ctl->plan[0].steps, ctl->plan[1].steps);
+  if (executed < 1 || ctl->acc_steps >= executed) return;  // block-uniform
+  const int p = executed & 1;  // the plan after the last step taken
+  const StepPlan pl = ctl->plan[p];
+  const unsigned bits = cfl_bits(P.cfl_slots + kCflSlots * p, executed);
+  const bool nonfinite = bits == kNonfiniteBits;
+  account_before<kFinishUnroll>(ctl, L, P, (1 - p) * P.stride, nparts, pl,
+                                __uint_as_float(nonfinite ? 0u : bits), nonfinite);
+  if (threadIdx.x == 0 && (pl.done || nonfinite)) ctl->done = 1;
+}
+
 template <class M, int MODE>
 __global__ __launch_bounds__(kBlock) void k_reduce(Grid G, const float* __restrict__ H,
                                                   const float* __restrict__ HU,
@@ -155,6 +239,18 @@ __global__ __launch_bounds__(kBlock) void k_reduce(Grid G, const float* __restri
   double t0 = 0.0;
   if (MODE == RED_INIT) t0 = L.times[0];
   const int rj = rain_bracket(L, t0);
+  if (MODE == RED_INIT) {
+    // Slots of the fused loop: the CFL maximum of the initial state, tagged
+    // step 0, and the other parity's cleared (tags of an earlier loop would
+    // outrank this one's).
+    __shared__ unsigned s_bits;
+    if (threadIdx.x == 0) s_bits = tot.nf ? kNonfiniteBits : __float_as_uint(tot.inv);
+    __syncthreads();
+    if (threadIdx.x < 2 * kCflSlots)
+      __hip_atomic_store(P.cfl_slots + threadIdx.x,
+                         threadIdx.x < kCflSlots ? (unsigned long long)s_bits : 0ull,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   if (threadIdx.x == 0) {
     ctl->max_inv = (double)tot.inv;
     if (MODE == RED_INIT) {
@@ -175,6 +271,18 @@ __global__ __launch_bounds__(kBlock) void k_reduce(Grid G, const float* __restri
         ctl->done = 0;
         plan_next_step(ctl, L, tot.inv, t0, oi, rj);
       }
+      // Plan of step 0 of the fused loop (the same clock, cursor and rate).
+      StepPlan& p0 = ctl->plan[0];
+      p0.t = t0;
+      p0.rate = interp_at(L.rain_t, L.rain_r, L.n_rain, L.rain_default, t0, rj);
+      for (int k = 0; k < 4; ++k) p0.tw[k] = oi + k < L.n_times ? L.times[oi + k] : HUGE_VAL;
+      p0.prev_dt = 0.0;
+      p0.prev_dtrate = 0.0f;
+      p0.steps = 0;
+      p0.oi = oi;
+      p0.done = ctl->done;
+      ctl->plan[1].steps = -1;
+      ctl->acc_steps = 0;
     }
     __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -384,9 +492,12 @@ struct SweContext {
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t graph = nullptr;
   std::vector<const void*> graph_key;
-  int split_finish = 1;      // finish each loop step in its own one-workgroup kernel
+  // How a device-loop step closes: 1 = fused (no finish kernel), 2 = a
+  // one-workgroup finish kernel after each step, 0 = the step's last workgroup.
+  int split_finish = 1;
   int alt_sweep = 1;         // odd device-loop steps sweep the strips in reverse order
   int sweep_rev = 0;         // reverse flag of the step launch being issued
+  int step_par = 0;          // parity of the step launch being issued (fused loop)
   double* dt_log = nullptr;  // caller's device buffer for the dt schedule
   int dt_log_cap = 0;
   float *ck_h = nullptr, *ck_hu = nullptr, *ck_hv = nullptr;  // caller's checkpoint cube
@@ -447,20 +558,26 @@ namespace {
 int ensure_partials(SweContext* c) {
   const int need = std::max(c->step_blocks(), c->reduce_blocks());
   if (need <= c->parts_cap) return SWE_OK;
-  if (c->P.vol) {
-    SWE_HIP_TRY(hipFree(c->P.vol));
+  if (c->P.cfl_slots) {
+    SWE_HIP_TRY(hipFree(c->P.cfl_slots));
     c->P = Partials{};
     c->parts_cap = 0;
   }
-  // One allocation: vol | clip | bnd (double) | inv (float) | nf (int).
-  const size_t bytes = (size_t)need * (3 * sizeof(double) + sizeof(float) + sizeof(int));
+  // One allocation: the fused loop's 2 x 64 CFL slots, then two parities of
+  // need partials each: vol | clip | bnd (double) | inv (float) | nf (int).
+  const size_t n2 = 2 * (size_t)need;
+  const size_t bytes = 2 * kCflSlots * sizeof(unsigned long long) +
+                       n2 * (3 * sizeof(double) + sizeof(float) + sizeof(int));
   char* base = nullptr;
   SWE_HIP_TRY(hipMalloc(&base, bytes));
-  c->P.vol = reinterpret_cast<double*>(base);
-  c->P.clip = c->P.vol + need;
-  c->P.bnd = c->P.clip + need;
-  c->P.inv = reinterpret_cast<float*>(c->P.bnd + need);
-  c->P.nf = reinterpret_cast<int*>(c->P.inv + need);
+  SWE_HIP_TRY(hipMemset(base, 0, bytes));
+  c->P.cfl_slots = reinterpret_cast<unsigned long long*>(base);
+  c->P.vol = reinterpret_cast<double*>(c->P.cfl_slots + 2 * kCflSlots);
+  c->P.clip = c->P.vol + n2;
+  c->P.bnd = c->P.clip + n2;
+  c->P.inv = reinterpret_cast<float*>(c->P.bnd + n2);
+  c->P.nf = reinterpret_cast<int*>(c->P.inv + n2);
+  c->P.stride = need;
   c->parts_cap = need;
   return SWE_OK;
 }
@@ -533,13 +650,25 @@ void launch_step(SweContext* c, const float* h, const float* hu, const float* hv
                  float* ch, float* chu, float* chv, hipStream_t s) {
   hipLaunchKernelGGL((k_step<M, MODE, PACKED>), dim3(c->step_blocks()), dim3(kBlock), c->step_lds, s, c->G, h, hu,
                      hv, ho, huo, hvo, dt, dtrate, dtg, c->rows, c->segs_x(), c->n_waves(),
-                     c->sweep_rev, c->d_ctl, L, c->P, ch, chu, chv);
+                     c->sweep_rev, c->step_par, c->d_ctl, L, c->P, ch, chu, chv);
 }
 
 void launch_step_any(SweContext* c, int mode, const float* h, const float* hu, const float* hv,
                      float* ho, float* huo, float* hvo, float dt, float dtrate, float dtg,
                      const LoopConst& L, float* ch, float* chu, float* chv, hipStream_t s) {
-  if (mode == MODE_LOOP && c->split_finish) {
+  if (mode == MODE_LOOP && c->split_finish == 1) {
+    if (c->loop_packed)
+      launch_step<FastMath, MODE_LOOP_FUSED, true>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L,
+                                                   ch, chu, chv, s);
+    else if (c->variant == SWE_VARIANT_REFERENCE)
+      launch_step<RefMath, MODE_LOOP_FUSED>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch,
+                                            chu, chv, s);
+    else
+      launch_step<FastMath, MODE_LOOP_FUSED>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L, ch,
+                                             chu, chv, s);
+    return;
+  }
+  if (mode == MODE_LOOP && c->split_finish == 2) {
     if (c->loop_packed)
       launch_step<FastMath, MODE_LOOP_SPLIT, true>(c, h, hu, hv, ho, huo, hvo, dt, dtrate, dtg, L,
                                                    ch, chu, chv, s);
@@ -610,9 +739,13 @@ double host_interp(const SweContext* c, double t) {
 void launch_loop_step(SweContext* c, int a, const LoopConst& L, float* ch, float* chu, float* chv,
                       hipStream_t s) {
   c->sweep_rev = c->alt_sweep ? a : 0;
+  c->step_par = a;
   struct Reset {
     SweContext* c;
-    ~Reset() { c->sweep_rev = 0; }
+    ~Reset() {
+      c->sweep_rev = 0;
+      c->step_par = 0;
+    }
   } reset{c};
   if (c->loop_packed)
     launch_step_any(c, MODE_LOOP, c->ws(a), nullptr, nullptr, c->ws(1 - a), nullptr, nullptr, 0.f,
@@ -662,6 +795,16 @@ int launch_loop_chunk(SweContext* c, int count, const LoopConst& L, float* ch, f
   return SWE_OK;
 }
 
+// After the last step kernel of a fused loop: account the last step taken
+// (k_close); the control block then holds the loop's final clock, step count,
+// output cursor and FP64 balance, as after a finish kernel.
+int close_loop(SweContext* c, const LoopConst& L, hipStream_t s) {
+  if (c->split_finish != 1) return SWE_OK;
+  hipLaunchKernelGGL(k_close, dim3(1), dim3(kBlock), 0, s, c->d_ctl, L, c->P, c->step_blocks());
+  SWE_HIP_TRY(hipGetLastError());
+  return SWE_OK;
+}
+
 // Run the device loop until the control block says done.  Chunks of steps
 // are queued two deep; after each chunk the control block is copied to a
 // pinned slot and the older chunk's copy decides whether to continue.  On
@@ -705,6 +848,7 @@ int drive_loop(SweContext* c, const LoopConst& L, float* ch, float* chu, float* 
       break;
     }
   }
+  if (rc == SWE_OK) rc = close_loop(c, L, s);
   e = hipStreamSynchronize(s);
   if (e != hipSuccess && rc == SWE_OK) rc = fail(SWE_EHIP, std::string("loop end: ") + hipGetErrorString(e));
   if (rc == SWE_OK) {
@@ -905,7 +1049,7 @@ int swe_destroy(SweContext* c) {
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->work) (void)hipFree(c->work);
   if (c->pk) (void)hipFree(c->pk);
-  if (c->P.vol) (void)hipFree(c->P.vol);
+  if (c->P.cfl_slots) (void)hipFree(c->P.cfl_slots);
   if (c->d_times) (void)hipFree(c->d_times);
   if (c->d_ctl) (void)hipFree(c->d_ctl);
   if (c->d_sum_mult) (void)hipFree(c->d_sum_mult);
@@ -966,7 +1110,8 @@ int swe_set_graph_chunk(SweContext* c, int32_t chunk) {
 }
 
 int swe_set_loop_finish(SweContext* c, int32_t split) {
-  if (!c || (split != 0 && split != 1)) return fail(SWE_EINVAL, "swe_set_loop_finish: split must be 0 or 1");
+  if (!c || split < 0 || split > 2) return fail(SWE_EINVAL, "swe_set_loop_finish: mode must be 0, 1 or 2");
+  if (c->loop_active) return fail(SWE_EINVAL, "swe_set_loop_finish: a device loop is active");
   c->split_finish = split;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
@@ -1239,6 +1384,7 @@ int swe_loop_end(SweContext* c, SweState state, SweRunInfo* info, void* stream) 
   hipStream_t s = (hipStream_t)stream;
   int rc;
   c->loop_active = false;
+  if ((rc = close_loop(c, c->loop_L, s))) return rc;
   if ((rc = copy_final(c, state, s))) return rc;
   if (info) {
     SWE_HIP_TRY(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(LoopControl), hipMemcpyDeviceToHost, s));

@@ -125,10 +125,15 @@ int swe_set_waves_per_cu(SweContext* ctx, int32_t waves);
  * the same size rule as swe_set_tile_rows).  No
  * reference counterpart (the reference differentiates with jax.grad). */
 int swe_set_adjoint_rows(SweContext* ctx, int32_t rows);
-/* How the device loop closes a step (tuning knob): 1 (default) = every
- * workgroup stores its CFL/mass partials and a one-workgroup kernel reduces
- * them and plans the next step; 0 = the last workgroup of the step kernel does
- * it (one launch per step, but a release/acquire across workgroups). */
+/* How the device loop closes a step (tuning knob).  1 (default) = fused:
+ * every workgroup folds its CFL maximum into 64 slots by atomic max and
+ * stores its FP64 mass partials; every workgroup of the next step derives dt
+ * from the slots itself, and that step's workgroup 0 plans the step after and
+ * accounts the one before (one launch per step, no finish kernel).  2 = every
+ * workgroup stores its partials and a one-workgroup kernel reduces them and
+ * plans the next step.  0 = the last workgroup of the step kernel does it
+ * (one launch per step, but a release/acquire across workgroups).  All three
+ * give the same states, step sizes and FP64 balance. */
 int swe_set_loop_finish(SweContext* ctx, int32_t split);
 /* Strip order of the device loop (tuning knob): 1 (default) = odd steps
  * sweep the strips in reverse, so each step starts where the previous one
