@@ -2,21 +2,36 @@
 
 Workload: the 4096 x 4096 FP32 urban dam-break case (dx = 1 m, wet start
 h = 0.05 m with a 0.5 m square in the centre, no rain, cfl 0.45, one open edge),
-advanced by the device-resident adaptive loop: each step is one fused kernel
-(ghosts, reconstruction, HLL, rain, clip, friction, wet/dry mask) whose
-epilogue writes per-workgroup CFL maxima and FP64 mass partials, followed by a
-one-workgroup finish kernel that sets the next dt; chunks of steps are HIP
-graphs.  Inputs are resident in HBM before timing starts.
+advanced by the device-resident adaptive loop: one fused kernel per step
+(ghosts, reconstruction, HLL, rain, clip, friction, wet/dry mask; its
+epilogue folds the CFL maximum of the new state into atomic slots and stores
+FP64 mass partials; every workgroup of the next step derives dt from the
+slots).  Inputs are resident in HBM before timing starts.
 
-Also reported: the batched ensemble (64 uneven urban members, 3600 s each)
-and the checkpointed adjoint (1024^2 urban, dry start with rain, ~2000 steps,
-with a frozen-schedule finite-difference check) against the plain step.
+Timing protocol: the tile autotune, then ``--settle`` untimed steps of the
+same loop, ``--warmup`` untimed steps, then exactly ``--steps`` timed steps in
+``--segments`` segments, all queued back to back with no host synchronisation
+in between.  The settle steps are there because the GPU's power management
+slows steps 20-150 after an idle gap by 10-25 % (measured: constant cycles and
+instructions per dispatch, so not the kernel); without them the timed region
+of a short run would fall inside that transient.  HIP events on the stream the
+kernels run on bracket every segment; ``value`` = cell-updates/s over all
+timed steps, ``segments`` gives each segment's ms per step and their median.
 
-Prints ONE JSON line.  ``value`` = cell-updates/s over exactly ``--steps``
-timed steps (HIP events on the stream the kernels run on, bracketed by
-device synchronisation); ``roofline`` prices the step at 37 algorithmic bytes
-per cell-update against 8 TB/s; ``cpu_baseline`` times the NumPy FP32 oracle
-on a few steps of the same workload on the host (one thread).
+Also reported: the in-house streaming probes (copy, read, and the packed
+step's streams without its arithmetic) as the achievable-bandwidth ceiling;
+the batched ensemble (64 uneven urban members, 3600 s each) and the
+checkpointed adjoint (1024^2 urban, dry start with rain, ~2000 steps, with a
+frozen-schedule finite-difference check) against the plain step; the FP64
+mass balance of each benchmark configuration; and, in the ``cpu_baseline``
+leg, the NumPy FP32 oracle timed on the host (a few steps of the 4096^2
+workload, and the 256^2 sloped-plane configuration in full, median of reps)
+together with the accuracy record of the GPU's run of that configuration
+against the oracle (per-field max-abs and RMSE against the FP64 oracle,
+wet-mask mismatches against the FP32 oracle).
+
+Prints ONE JSON line.  ``roofline`` prices the step at 37 algorithmic bytes
+per cell-update against 8 TB/s.
 
     python bench.py [--steps 2000] [--warmup 20] [--variant fast|reference]
 """
@@ -42,13 +57,21 @@ def _even_divisor(k: int, cap: int = 64) -> int:
     for c in range(min(cap, k), 1, -1):
         if k % c == 0 and c % 2 == 0:
             return c
-    return 2
+    return 0
 
 
-def time_loop(params, h0, steps, warmup, variant, rows=None):
-    """Advance the device loop: ``warmup`` untimed steps, then ``steps`` timed."""
+def _segment_sizes(steps: int, segments: int) -> list[int]:
+    n = max(1, min(segments, steps))
+    base, extra = divmod(steps, n)
+    return [base + (1 if k < extra else 0) for k in range(n)]
+
+
+def time_loop(params, h0, steps, warmup, settle, segments, variant, rows=None):
+    """Advance the device loop: ``settle`` + ``warmup`` untimed steps, then
+    ``steps`` timed steps in segments, queued without host synchronisation."""
     import ctypes
 
+    import numpy as np
     import torch
 
     from hybrid_flood_amd import _lib
@@ -66,40 +89,120 @@ def time_loop(params, h0, steps, warmup, variant, rows=None):
         h = torch.as_tensor(h0, device=dev).contiguous()
         tuned = tune_rows(params, nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h)),
                           variant=variant)
-    chunk = _even_divisor(steps)
+    sizes = _segment_sizes(steps, segments)
+    chunk = _even_divisor(sizes[0])
     ctx.set_graph_chunk(chunk)
-    warm = max(chunk, int(math.ceil(warmup / chunk)) * chunk) if warmup > 0 else 0
     h = torch.as_tensor(h0, device=dev).contiguous()
     state = nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h))
     s = _lib.stream_handle(dev)
     st = _lib.state_struct(*state)
-    _lib.check(lib.swe_loop_begin(ctx.handle, st, 0.0, warm + steps, s), "swe_loop_begin")
-    if warm:
-        _lib.check(lib.swe_loop_advance(ctx.handle, warm, s), "swe_loop_advance")
+    total = settle + warmup + steps
+    _lib.check(lib.swe_loop_begin(ctx.handle, st, 0.0, total, s), "swe_loop_begin")
     torch.cuda.synchronize(dev)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    ev0.record()
-    _lib.check(lib.swe_loop_advance(ctx.handle, steps, s), "swe_loop_advance")
-    ev1.record()
+    if settle:
+        _lib.check(lib.swe_loop_advance(ctx.handle, settle, s), "swe_loop_advance")
+    if warmup:
+        _lib.check(lib.swe_loop_advance(ctx.handle, warmup, s), "swe_loop_advance")
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(sizes) + 1)]
+    evs[0].record()
+    for k, n in enumerate(sizes):
+        _lib.check(lib.swe_loop_advance(ctx.handle, n, s), "swe_loop_advance")
+        evs[k + 1].record()
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - t0
-    ms = ev0.elapsed_time(ev1)
+    ms = evs[0].elapsed_time(evs[-1])
+    seg = [evs[k].elapsed_time(evs[k + 1]) / n for k, n in enumerate(sizes)]
     info = _lib.SweRunInfo()
     _lib.check(lib.swe_loop_end(ctx.handle, st, ctypes.byref(info), s), "swe_loop_end")
-    if info.steps_used != warm + steps:
-        raise RuntimeError(f"device loop took {info.steps_used} steps, expected {warm + steps}")
-    return {"ms": ms, "wall_s": wall, "warmup": warm, "info": info.as_dict(), "chunk": chunk,
+    if info.steps_used != total:
+        raise RuntimeError(f"device loop took {info.steps_used} steps, expected {total}")
+    return {"ms": ms, "wall_s": wall, "info": info.as_dict(), "chunk": chunk,
+            "segments": {"count": len(sizes), "steps": sizes,
+                         "ms_per_step": seg, "median_ms_per_step": float(np.median(seg))},
             "tile": ({"rows": tuned["rows"], "waves_per_cu": tuned["waves_per_cu"], "autotuned": True}
                      if tuned else {"rows": rows, "waves_per_cu": 0, "autotuned": False})}
 
 
-def cpu_baseline(case, steps):
-    """NumPy FP32 oracle on ``steps`` adaptive steps of the same workload (one thread)."""
+def stream_probes(cells: int, reps: int = 10) -> dict:
+    """Achievable bandwidth of this device, measured with the library's own
+    streaming kernels: a copy, a read, and the packed step's three streams
+    (12 B + 8 B read, 12 B written per cell) over the step's cell count."""
+    import ctypes
+
+    import torch
+
+    from hybrid_flood_amd import _lib
+
+    lib = _lib.load()
+    s = _lib.stream_handle()
+    out = {}
+    n = 128 * 1024 * 1024  # floats: 512 MiB per buffer
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+
+    def run(mode, src, src2, dst, count, nbytes):
+        for _ in range(3):
+            _lib.check(lib.swe_stream_probe(mode, src, src2, dst, count, s), "swe_stream_probe")
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            _lib.check(lib.swe_stream_probe(mode, src, src2, dst, count, s), "swe_stream_probe")
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        return {"us_per_launch": ms * 1e3, "GB_s": nbytes / (ms * 1e-3) / 1e9,
+                "frac_of_8TBs": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+    vp = ctypes.c_void_p
+    out["copy"] = run(0, vp(a.data_ptr()), None, vp(b.data_ptr()), n, 8 * n)
+    out["read"] = run(1, vp(a.data_ptr()), None, vp(b.data_ptr()), n, 4 * n)
+    del a, b
+    st = torch.ones(3 * cells, device="cuda")
+    sx = torch.ones(2 * cells, device="cuda")
+    dst = torch.empty(3 * cells, device="cuda")
+    out["step_streams"] = run(2, vp(st.data_ptr()), vp(sx.data_ptr()), vp(dst.data_ptr()), cells,
+                              32 * cells)
+    out["step_streams"]["note"] = ("the packed step's streams (32 B per cell) without its "
+                                   "arithmetic: the memory-only time of one step")
+    return out
+
+
+def config1_gpu_run(variant: str) -> dict:
+    """The 256^2 sloped-plane configuration (dx 2 m, dry start, trapezoid
+    rain, cfl 0.9, 600 s, outputs every 60 s) on the GPU: output states, the
+    loop's own dt schedule and its FP64 mass balance."""
+    import torch
+
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd import synthetic as sy
+    from hybrid_flood_amd.adjoint import rollout_forward
+
+    dev = torch.device("cuda", 0)
+    c = sy.sloped_bumps_case(256)
+    params = nm.make_params(c.bed, c.manning_n, c.domain_mask, boundary_types=c.boundary_types,
+                            rainfall_times_s=c.rain_times_s, rainfall_rates_m_s=c.rain_rates_m_s,
+                            dx=c.dx, cfl=c.cfl, dry_tolerance=c.dry_tolerance, max_dt=c.max_dt,
+                            device=dev)
+    h = torch.as_tensor(c.h0, device=dev)
+    times = c.output_times()
+    _, res, tape = rollout_forward(params, nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h)),
+                                   times, max_steps=20000, variant=variant, record=False)
+    return {"case": c, "times": times, "states": [x.cpu().numpy() for x in res.states],
+            "dts": tape.dts, "steps": int(tape.steps), "diagnostics": res.diagnostics}
+
+
+def cpu_baseline(case, steps: int, cfg1: dict, reps: int) -> tuple[dict, dict]:
+    """The NumPy FP32 oracle on the host, one thread: ``steps`` adaptive steps
+    of the 4096^2 workload (the reported value), and the 256^2 configuration
+    in full (median of ``reps``).  The oracle is also the checker of the
+    GPU's run of that configuration: replayed in float32 and float64 along the
+    GPU's own dt schedule, it gives the accuracy record."""
     import numpy as np
 
     sys.path.insert(0, ROOT)
+    from oracle import accuracy as acc
     from oracle import swe_oracle as o
 
     p = o.OracleParams(bed=case.bed, manning_n=case.manning_n, domain_mask=case.domain_mask,
@@ -111,12 +214,65 @@ def cpu_baseline(case, steps):
     o.run_steps(o.OracleState(case.h0, z, z), p, steps)
     el = time.perf_counter() - t0
     n = case.bed.size
-    return {
+    c1 = cfg1["case"]
+    op1 = o.OracleParams(bed=c1.bed, manning_n=c1.manning_n, domain_mask=c1.domain_mask,
+                         boundary_types=c1.boundary_types, rainfall_times_s=c1.rain_times_s,
+                         rainfall_rates_m_s=c1.rain_rates_m_s, dx=c1.dx, cfl=c1.cfl,
+                         dry_tolerance=c1.dry_tolerance, max_dt=c1.max_dt)
+    z1 = np.zeros_like(c1.h0)
+    secs = []
+    r1 = None
+    for _ in range(reps):
+        t1 = time.perf_counter()
+        r1 = o.integrate_to_outputs(o.OracleState(c1.h0, z1, z1), op1, cfg1["times"],
+                                    max_steps=20000)
+        secs.append(time.perf_counter() - t1)
+    med = float(np.median(secs))
+    base = {
         "value": n * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
         "sample": f"{steps} adaptive steps (CFL dt + step) of the same {case.bed.shape[0]}x"
                   f"{case.bed.shape[1]} workload, NumPy FP32 oracle, single thread, {el:.1f} s",
         "host_cpu_count": os.cpu_count(),
+        "config1_full": {
+            "workload": "sloped plane with bumps and a channel, 256x256, dx 2 m, dry start, "
+                        "trapezoid rain to 50 mm/h, cfl 0.9, 600 s, 11 outputs",
+            "steps": int(r1.steps_used), "reps": reps, "seconds": secs, "median_s": med,
+            "cell_updates_per_s": c1.bed.size * r1.steps_used / med,
+        },
     }
+    rec = acc.record(cfg1["states"], op1, c1.h0, cfg1["times"], cfg1["dts"])
+    rec["workload"] = "256x256 sloped plane (config 1), 600 s, 11 outputs, GPU fast variant"
+    rec["gpu_steps"] = cfg1["steps"]
+    rec["oracle_own_schedule_steps"] = int(r1.steps_used)
+    rec["gpu_mass_balance_rel"] = cfg1["diagnostics"]["mass_balance_rel"]
+    return base, rec
+
+
+def config2_mass_balance(variant: str) -> dict:
+    """The 1024^2 urban configuration (one open edge, 1-min rain series over
+    10800 s, 181 outputs) on the GPU: steps, wall time and FP64 mass balance."""
+    import numpy as np
+    import torch
+
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd import synthetic as sy
+
+    dev = torch.device("cuda", 0)
+    u = sy.urban_case(1024)
+    params = nm.make_params(u.bed, u.manning_n, u.domain_mask, boundary_types=u.boundary_types,
+                            rainfall_times_s=u.rain_times_s, rainfall_rates_m_s=u.rain_rates_m_s,
+                            dx=u.dx, cfl=u.cfl, dry_tolerance=u.dry_tolerance, max_dt=u.max_dt,
+                            device=dev)
+    h = torch.as_tensor(u.h0, device=dev)
+    times = np.arange(0.0, 10801.0, 60.0).astype(np.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = nm.integrate_to_outputs(nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h)), params,
+                                times, max_steps=200000, variant=variant)
+    wall = time.perf_counter() - t0
+    d = r.diagnostics
+    return {"outputs_written": int(r.outputs_written), "steps": int(r.steps_used), "wall_s": wall,
+            "mass_balance_rel": d["mass_balance_rel"], "nonfinite": d["nonfinite"]}
 
 
 def bench_ensemble(variant: str, count: int, seed: int, t_end: float, plain_cups: float):
@@ -281,7 +437,8 @@ def bench_adjoint(variant: str, n: int, steps: int, plain_cups: float):
         "gradient_vs_forward_time": grad_s / fwd_s,
         "gradient_cell_steps_per_s": cells * K / grad_s,
         "vs_plain_step": (cells * K / grad_s) / plain_cups,
-        "loss": float(loss),
+        "loss": float(loss.detach()),
+        "forward_mass_balance_rel": res.diagnostics["mass_balance_rel"],
         "fd_check": {
             "d_loss_d_rain_multiplier": {"adjoint": g_a, "central_fd": fd_a, "eps": eps_a,
                                          "rel_err": abs(g_a - fd_a) / max(abs(fd_a), 1e-300)},
@@ -388,10 +545,16 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--settle", type=int, default=400,
+                    help="untimed steps before the warm-up, so the timed region does not start "
+                         "in the GPU's power-management transient after an idle gap")
+    ap.add_argument("--segments", type=int, default=10, help="event-bracketed timed segments")
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--variant", default="fast", choices=["fast", "reference"])
-    ap.add_argument("--rows", type=int, default=0, help="rows per wavefront strip (0 = default)")
+    ap.add_argument("--rows", type=int, default=0, help="rows per wavefront strip (0 = autotune)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-reps", type=int, default=5,
+                    help="timed repetitions of the oracle on the 256^2 configuration")
     ap.add_argument("--extra", action="store_true", help="also time the other kernel variant")
     ap.add_argument("--ensemble", type=int, default=64, help="ensemble members (0 = skip)")
     ap.add_argument("--ensemble-t", type=float, default=3600.0, help="simulated seconds per member")
@@ -399,6 +562,9 @@ def main(argv=None) -> int:
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--callers", type=int, default=1,
                     help="time the coupled-forecast, staging and output kernels (0 = skip)")
+    ap.add_argument("--probes", type=int, default=1, help="streaming bandwidth probes (0 = skip)")
+    ap.add_argument("--configs", type=int, default=1,
+                    help="run configurations 1 and 2 for their mass balance (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 device copies of 256 MiB (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -410,8 +576,9 @@ def main(argv=None) -> int:
               "(the step is measured on one MI355X; multi-GPU slab sharding is design only).",
               file=sys.stderr)
         return 2
-    if args.steps < 2:
-        print("bench.py: --steps must be >= 2", file=sys.stderr)
+    if args.steps < 1 or args.warmup < 0 or args.settle < 0 or args.segments < 1:
+        print("bench.py: --steps >= 1, --warmup >= 0, --settle >= 0, --segments >= 1",
+              file=sys.stderr)
         return 2
 
     import torch
@@ -427,7 +594,8 @@ def main(argv=None) -> int:
                             rainfall_rates_m_s=case.rain_rates_m_s, dx=case.dx, cfl=case.cfl,
                             dry_tolerance=case.dry_tolerance, max_dt=case.max_dt, device=dev)
     cells = case.bed.size
-    r = time_loop(params, case.h0, args.steps, args.warmup, args.variant, args.rows)
+    r = time_loop(params, case.h0, args.steps, args.warmup, args.settle, args.segments,
+                  args.variant, args.rows)
     ms_per_step = r["ms"] / args.steps
     value = cells * args.steps / (r["ms"] * 1e-3)
     achieved = BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9
@@ -437,7 +605,7 @@ def main(argv=None) -> int:
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": r["warmup"],
+        "warmup": args.warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -462,6 +630,8 @@ def main(argv=None) -> int:
             "traffic": None,
             "bytes_per_cell_update": BYTES_PER_CELL,
         },
+        "settle_steps": args.settle,
+        "segments": r["segments"],
         "wall_s": r["wall_s"],
         "mass_balance_rel": r["info"]["mass_balance_rel"],
         "simulated_s": r["info"]["final_time_s"],
@@ -472,21 +642,38 @@ def main(argv=None) -> int:
             t = json.load(f)
         line["roofline"]["traffic"] = t.get("hbm_bytes_per_launch")
         line["roofline"]["traffic_source"] = "profiles/step_fast_4096_traffic.json (rocprofv3 --pmc)"
+    if args.probes > 0:
+        line["bandwidth_probes"] = stream_probes(cells)
     if args.extra:
         other = "reference" if args.variant == "fast" else "fast"
         k = min(args.steps, 400)
-        ro = time_loop(params, case.h0, k, args.warmup, other, args.rows)
+        ro = time_loop(params, case.h0, k, args.warmup, args.settle, args.segments, other,
+                       args.rows)
         line["other_variant"] = {"variant": other, "steps": k,
                                  "cell_updates_per_s": cells * k / (ro["ms"] * 1e-3),
                                  "ms_per_step": ro["ms"] / k}
+    mb = {"config3_roofline_run": line["mass_balance_rel"]}
+    cfg1 = None
+    if args.configs > 0:
+        cfg1 = config1_gpu_run(args.variant)
+        mb["config1"] = cfg1["diagnostics"]["mass_balance_rel"]
+        c2 = config2_mass_balance(args.variant)
+        mb["config2"] = c2["mass_balance_rel"]
+        line["config2_run"] = c2
     if args.ensemble > 0:
         line["ensemble"] = bench_ensemble(args.variant, args.ensemble, 42, args.ensemble_t, value)
+        mb["config4_max_abs_over_members"] = line["ensemble"]["max_abs_mass_balance_rel"]
     if args.adjoint_n > 0:
         line["adjoint"] = bench_adjoint(args.variant, args.adjoint_n, args.adjoint_steps, value)
+        mb["config5_forward"] = line["adjoint"]["forward_mass_balance_rel"]
+    line["mass_balance_rel_by_config"] = mb
     if args.callers > 0:
         line["callers"] = bench_callers(params, case)
     if args.cpu_steps > 0:
-        line["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+        if cfg1 is None:
+            cfg1 = config1_gpu_run(args.variant)
+        line["cpu_baseline"], line["accuracy"] = cpu_baseline(case, args.cpu_steps, cfg1,
+                                                              args.cpu_reps)
     print(json.dumps(line), flush=True)
     return 0
 

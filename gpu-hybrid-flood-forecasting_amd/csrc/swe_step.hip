@@ -506,6 +506,7 @@ struct SweContext {
   double loop_times[2] = {0.0, 0.0};
   LoopConst loop_L{};
   bool loop_active = false;
+  int loop_launch_par = 0;  // parity of the next step launch of swe_loop_advance
   double* aux = nullptr;  // scratch of the coupled-forecast kernels
   size_t aux_cap = 0;
   int lds_per_cu = 0;        // bytes of LDS per compute unit
@@ -1366,6 +1367,7 @@ int swe_loop_begin(SweContext* c, SweState state, double t0, int32_t max_steps, 
   launch_reduce(c, RED_INIT, state.h, state.hu, state.hv, c->loop_L, s);
   SWE_HIP_TRY(hipGetLastError());
   c->loop_active = true;
+  c->loop_launch_par = 0;
   return SWE_OK;
 }
 
@@ -1373,9 +1375,26 @@ int swe_loop_advance(SweContext* c, int32_t n_launches, void* stream) {
   if (!c || !c->loop_active) return fail(SWE_EINVAL, "swe_loop_advance: call swe_loop_begin first");
   if (n_launches < 0) return fail(SWE_EINVAL, "swe_loop_advance: n_launches must be >= 0");
   if (n_launches == 0) return SWE_OK;
-  const int chunk = c->graph_chunk > 0 ? c->graph_chunk : 2;
-  const int launches = ((n_launches + chunk - 1) / chunk) * chunk;
-  return launch_loop_chunk(c, launches, c->loop_L, nullptr, nullptr, nullptr, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  // Exactly n_launches step kernels.  Launch k reads work state k & 1, so a
+  // call that starts on an odd launch issues one plain launch first; whole
+  // graph chunks (which start on an even launch) follow, then plain launches
+  // for the remainder.
+  int left = n_launches;
+  if (c->loop_launch_par == 1) {
+    launch_loop_step(c, 1, c->loop_L, nullptr, nullptr, nullptr, s);
+    c->loop_launch_par = 0;
+    --left;
+  }
+  if (c->graph_chunk > 0 && left >= c->graph_chunk) {
+    const int full = (left / c->graph_chunk) * c->graph_chunk;
+    if (int rc = launch_loop_chunk(c, full, c->loop_L, nullptr, nullptr, nullptr, s)) return rc;
+    left -= full;
+  }
+  for (int k = 0; k < left; ++k) launch_loop_step(c, k & 1, c->loop_L, nullptr, nullptr, nullptr, s);
+  c->loop_launch_par = left & 1;
+  SWE_HIP_TRY(hipGetLastError());
+  return SWE_OK;
 }
 
 int swe_loop_end(SweContext* c, SweState state, SweRunInfo* info, void* stream) {

@@ -142,6 +142,7 @@ SIGNATURES = {
         [c_void_p, SweState, c_void_p, c_int64, c_int64, c_float, POINTER(c_float), SweState,
          POINTER(c_double), c_void_p],
     ),
+    "swe_stream_probe": (c_int32, [c_int32, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "swe_conserved_to_primitive": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,

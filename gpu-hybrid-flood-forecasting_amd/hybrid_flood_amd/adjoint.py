@@ -59,6 +59,7 @@ class RolloutModel:
     checkpoint_every: int = 0  # 0 = ceil(sqrt(max_steps)) recorded in the loop, else ceil(sqrt(K))
     record_checkpoints: bool = True  # store checkpoints during the forward loop (no replay pass)
     adjoint_precision: str = "f32"  # cotangent arithmetic of the reverse sweep: "f32" or "f64"
+    adjoint_pipeline: bool = True  # replay the next segment on a second stream during a reversal
 
     def params_for(self, manning_n: torch.Tensor, rain_multiplier) -> SWEParams:
         mult = (float(rain_multiplier) * self.base_multiplier.double()).float().contiguous()
@@ -241,6 +242,15 @@ def rollout_backward(tape: Tape, grad_depths: torch.Tensor, *, variant: str,
     main = torch.cuda.current_stream(dev)
     side = torch.cuda.Stream(device=dev) if len(cubes) > 1 else main
     side_handle = ctypes.c_void_p(side.cuda_stream) if len(cubes) > 1 else s
+    if side is not main:
+        # The side stream reads the checkpoints (filled on the main stream by
+        # the forward) and writes the cubes (allocated on the main stream,
+        # possibly over memory main-stream kernels still read): order all of
+        # that after the main stream's queued work, and tell the allocator the
+        # cubes are used on the side stream.
+        side.wait_stream(main)
+        for cube in cubes:
+            cube.record_stream(side)
     freed = [None] * len(cubes)  # event: the last reverse sweep that read each cube ended
 
     def replay(i):
@@ -310,7 +320,8 @@ class _RolloutFunction(torch.autograd.Function):
     def backward(ctx, grad_depths):
         model: RolloutModel = ctx.model
         gn, gm, _ = rollout_backward(ctx.tape, grad_depths.contiguous().float(), variant=model.variant,
-                                     precision=model.adjoint_precision)
+                                     precision=model.adjoint_precision,
+                                     pipeline=model.adjoint_pipeline)
         g_alpha = (gm.double() * model.base_multiplier.double()).sum()
         return gn.float(), g_alpha.to(ctx.mult_dtype), None
 

@@ -210,8 +210,9 @@ int swe_run_steps(SweContext* ctx, SweState state, int32_t n_steps, double t0,
  *   swe_loop_begin:   copy `state` into the context's work buffers, reduce its
  *                     CFL maximum and volume, prime the control block at t0
  *                     with a budget of max_steps steps (no output target).
- *   swe_loop_advance: queue n_launches step kernels (rounded up to the graph
- *                     chunk); launches after the budget is spent are no-ops.
+ *   swe_loop_advance: queue exactly n_launches step kernels (whole graph
+ *                     chunks where they fit, plain launches for the rest);
+ *                     launches after the budget is spent are no-ops.
  *   swe_loop_end:     copy the current state back to `state`; with `info`
  *                     non-NULL, synchronise and fill it.
  * All three are asynchronous on `stream` unless `info` is given. */
@@ -396,6 +397,20 @@ int swe_stage_features(int32_t count, int32_t ny, int32_t nx, const float* state
 int swe_correct_depth(int32_t count, int32_t ny, int32_t nx, const float* depth,
                       const float* residual, int64_t sample_stride, int64_t cell_stride,
                       float* out, void* stream);
+
+/* ---- Measurement ---------------------------------------------------------- */
+
+/* Streaming probe of the achievable HBM bandwidth (no counterpart in the
+ * reference, which never measured a device; the benchmark runs it beside the
+ * step).  Asynchronous on `stream`; time it with events.
+ *   mode 0: copy n floats src -> dst (n % 4 == 0, 16-byte aligned);
+ *   mode 1: read n floats of src (n % 4 == 0, 16-byte aligned), write one
+ *           float per 1024 read to dst (ceil(n / 1024) floats);
+ *   mode 2: the packed device loop's streams without its arithmetic: n cells,
+ *           src = 3n floats of packed state, src2 = 2n floats of packed static
+ *           fields, dst = 3n floats written. */
+int swe_stream_probe(int32_t mode, const void* src, const void* src2, void* dst, int64_t n,
+                     void* stream);
 
 /* ---- Output schema (jax_solver/shallow_water_2d.py:309-350,
  *      structured_io.py:160-211) ------------------------------------------ */

@@ -442,3 +442,30 @@ def test_pipelined_reverse_equals_serial(cuda, every, precision):
     piped = rollout_backward(tape, g, variant="fast", precision=precision, pipeline=True)
     for a, b in zip((serial[0], serial[1], *serial[2]), (piped[0], piped[1], *piped[2])):
         assert torch.equal(a, b)
+
+
+def test_pipelined_backward_right_after_forward(cuda):
+    """loss.backward() straight after rollout_depths, with the pipelined
+    reverse sweep: the side stream's first replay must wait for the forward's
+    checkpoints on the main stream.  Same gradients as the serial sweep."""
+    import dataclasses
+
+    import torch
+
+    from hybrid_flood_amd.adjoint import depth_rmse, rollout_depths
+
+    case, model = _rollout_setup(cuda)
+    target = torch.zeros((len(model.output_times_s),) + tuple(model.h0.h.shape), device=cuda)
+    n0 = torch.as_tensor(case["manning_n"], device=cuda)
+    grads = []
+    for pipeline in (True, False):
+        m = dataclasses.replace(model, adjoint_pipeline=pipeline, checkpoint_every=5)
+        n_var = n0.clone().requires_grad_(True)
+        alpha = torch.tensor(1.0, dtype=torch.float64, device=cuda, requires_grad=True)
+        loss = depth_rmse(rollout_depths(n_var, alpha, m), target)
+        loss.backward()
+        torch.cuda.synchronize()
+        grads.append((n_var.grad.clone(), alpha.grad.clone()))
+    assert torch.equal(grads[0][0], grads[1][0])
+    assert torch.equal(grads[0][1], grads[1][1])
+    assert float(grads[0][0].abs().sum()) > 0
