@@ -33,6 +33,7 @@ struct Grid {
   int nx, ny;
   int bt[4];  // left, right, bottom, top; 0 reflective, 1 open
   float g, half_g, dry, dx, dy, rdx, rdy;
+  float rho;  // dy / dx (the fast variant's CFL form)
   const float* __restrict__ bed;
   const float* __restrict__ n;
   const uint8_t* __restrict__ mask;
@@ -241,14 +242,27 @@ __device__ __forceinline__ CellUpdate cell_update(const Grid& G, float h, float 
 }
 
 // Inverse CFL time of one cell (numerics.py:299-303); 0 outside the mask.
+// The fast variant evaluates ((|u| + c) dy/dx + (|v| + c)) / dy with |u| =
+// |hu| / max(h, dry) as one fused multiply-add on the reciprocal, the form
+// the fast step's epilogue takes for the state it writes, so both give the
+// same bits for the same state.
 template <class M>
 __device__ __forceinline__ float cfl_inverse(const Grid& G, float h, float hu, float hv,
                                              int active) {
-  float u, v;
-  M::velocities(h, hu, hv, G.dry, u, v);
-  const float w = M::sqrt(G.g * fmaxf(h, 0.0f));
-  const float inv = M::per_dx(fabsf(u) + w, G.dx, G.rdx) + M::per_dx(fabsf(v) + w, G.dy, G.rdy);
-  return active ? inv : 0.0f;
+  if constexpr (M::kFast) {
+    const float r = __builtin_amdgcn_rcpf(fmaxf(h, G.dry));
+    const bool wet = h > G.dry;
+    const float w = M::sqrt(G.g * fmaxf(h, 0.0f));
+    const float su = fmaf(fabsf(wet ? hu : 0.0f), r, w);
+    const float sv = fmaf(fabsf(wet ? hv : 0.0f), r, w);
+    return active ? fmaf(su, G.rho, sv) * G.rdy : 0.0f;
+  } else {
+    float u, v;
+    M::velocities(h, hu, hv, G.dry, u, v);
+    const float w = M::sqrt(G.g * fmaxf(h, 0.0f));
+    const float inv = M::per_dx(fabsf(u) + w, G.dx, G.rdx) + M::per_dx(fabsf(v) + w, G.dy, G.rdy);
+    return active ? inv : 0.0f;
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -461,6 +461,7 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
     G.dy = (float)p.dy;
     G.rdx = 1.0f / G.dx;
     G.rdy = 1.0f / G.dy;
+    G.rho = (float)(p.dy / p.dx);
     G.bed = p.bed;
     G.n = p.manning_n;
     G.mask = p.domain_mask;

@@ -72,9 +72,9 @@ __device__ __forceinline__ FaceFlux face_fast(const Grid& G, float hA, float eta
                                               float unA, float utA, float pA, bool aA, float hB,
                                               float etaB, float bB, float unB, float utB,
                                               float pB, bool aB) {
+  const bool wallA = MASKED && aA && !aB;
+  const bool wallB = MASKED && !aA && aB;
   if (MASKED) {
-    const bool wallA = aA && !aB;
-    const bool wallB = !aA && aB;
     hB = wallA ? hA : hB;
     etaB = wallA ? etaA : etaB;
     bB = wallA ? bA : bB;
@@ -122,7 +122,13 @@ __device__ __forceinline__ FaceFlux face_fast(const Grid& G, float hA, float eta
   const float kb = -sL * r;         // weight of the B-side flux
   const float kc = (sL * sR) * r;   // weight of the state jump
   FaceFlux F;
-  F.f0 = fmaf(kc, hrB - hrA, ka * QnA + kb * QnB);
+  // The mass flux of a wall face (one side mirrored: hrB = hrA, QnB = -QnA,
+  // ka = kb) must be exactly zero: two separately rounded products cancel
+  // there, a fused one would leave its rounding error.  Every other face
+  // takes the fused form, in the masked and the all-active code alike, so a
+  // face gives the same bits whichever path its wavefront takes.
+  const float mass = fmaf(ka, QnA, kb * QnB);
+  F.f0 = fmaf(kc, hrB - hrA, MASKED && (wallA || wallB) ? ka * QnA + kb * QnB : mass);
   F.fn = fmaf(kc, QnB - QnA, fmaf(ka, FnA, kb * FnB));
   F.ft = fmaf(kc, dQt, fmaf(ka, FtA, kb * FtB));
   F.cA = pA - prA;
@@ -524,11 +530,14 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
             cube_hv[(size_t)k * N + idx] = hv_o;
           }
         }
-        // CFL of the new state (numerics.py:299-303): u = hu/max(h,dry) where
-        // wet, and hu is already 0 where not wet.
+        // CFL of the new state (numerics.py:299-303), u = hu/max(h,dry) where
+        // wet (hu is already 0 where not wet), in units of 1/dy:
+        // (|u| + c) dy/dx + (|v| + c); the lane's maximum is scaled by 1/dy
+        // once, after the march (cfl_inverse<FastMath> takes the same form).
         const float ws = __builtin_amdgcn_sqrtf(G.g * h_o);
-        const float inv = fmaf(fabsf(hv_o * rhd) + ws, rdy, (fabsf(hu_o * rhd) + ws) * rdx);
-        acc.inv = max_nc(acc.inv, ca ? inv : 0.0f);
+        const float su = fmaf(fabsf(hu_o), rhd, ws);
+        const float sv = fmaf(fabsf(hv_o), rhd, ws);
+        acc.inv = max_nc(acc.inv, ca ? fmaf(su, G.rho, sv) : 0.0f);
         acc.vol += (double)h_o;
         clip32 += ca ? hn - hs : 0.0f;  // max(-hs, 0), exactly
         // (x + y + z) * 0 stays 0 unless one of them is infinite or NaN
@@ -573,6 +582,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   if (LOOP) {
     acc.nf |= !isfinite(nf_probe);
     acc.clip += (double)clip32;
+    acc.inv *= G.rdy;  // max(x) / dy = max(x / dy): rounding is monotonic
   }
 }
 

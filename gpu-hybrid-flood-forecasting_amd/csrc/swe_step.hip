@@ -1006,6 +1006,7 @@ int swe_create(const SweParamsDesc* p, int32_t variant, SweContext** out) {
   G.dy = (float)p->dy;
   G.rdx = 1.0f / G.dx;
   G.rdy = 1.0f / G.dy;
+  G.rho = (float)(p->dy / p->dx);
   G.bed = p->bed;
   G.n = p->manning_n;
   G.mask = p->domain_mask;

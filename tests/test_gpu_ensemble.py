@@ -65,3 +65,96 @@ def test_ensemble_respects_max_steps(cuda):
     ens = EnsembleRunner(members)
     with pytest.raises(RuntimeError):
         ens.run(1000.0, max_steps=7)
+
+
+def _config4_members(cuda, t_end):
+    """The 64 members of the benchmark ensemble: sides from {256, 512, 1024,
+    2048} (seed 42), each its own urban DEM at dx 2 m, Manning n x U[0.7, 1.3],
+    rain x U[0.5, 2.0]."""
+    from hybrid_flood_amd import synthetic as sy
+    from hybrid_flood_amd.ensemble import member_from_case
+
+    members, sides = [], []
+    for side, mseed, n_scale, rain_scale in sy.ensemble_members(64, 42):
+        case = sy.urban_case(side, dx=2.0, seed=mseed, duration_s=t_end)
+        members.append(member_from_case(case, device=cuda, n_scale=n_scale, rain_scale=rain_scale))
+        sides.append(side)
+    return members, sides
+
+
+def test_config4_ensemble_64_uneven_members(cuda):
+    """The whole benchmark ensemble over 3600 s: every member reaches t_end
+    with a finite state, and one member of each side length equals its solo
+    run through the single-domain device loop bit for bit."""
+    import torch
+
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd.ensemble import EnsembleRunner
+
+    t_end = 3600.0
+    members, sides = _config4_members(cuda, t_end)
+    assert sorted(set(sides)) == [256, 512, 1024, 2048]
+    pick = {s: sides.index(s) for s in sorted(set(sides))}
+    start = {k: [x.clone() for x in members[k].state] for k in pick.values()}
+    ens = EnsembleRunner(members, variant="fast")
+    print("config 4: 64 members, 3600 s", flush=True)
+    infos = ens.run(t_end, max_steps=200_000)
+    torch.cuda.synchronize()
+    for k, info in enumerate(infos):
+        assert info["nonfinite"] == 0, k
+        assert info["final_time_s"] == pytest.approx(t_end, abs=1e-9), k
+        assert info["outputs_written"] == 2, k
+        # float32 state over ~2000-10000 steps (the bench records 3.1e-6 at worst)
+        assert abs(info["mass_balance_rel"]) < 1e-5, k
+        assert bool(torch.isfinite(members[k].state.h).all()) and float(members[k].state.h.min()) >= 0
+    for side, k in pick.items():
+        m = members[k]
+        r = nm.integrate_to_outputs(nm.SWEState(*start[k]), m.params,
+                                    np.array([0.0, t_end], np.float32), max_steps=200_000,
+                                    variant="fast")
+        assert r.steps_used == infos[k]["steps_used"], side
+        for got, want in zip(m.state, r.states):
+            assert torch.equal(got, want[-1]), f"member {k} ({side}^2) differs from its solo run"
+
+
+def test_config4_member_256_against_fp32_oracle(cuda):
+    """A 256^2 member of the benchmark ensemble (its own DEM, scaled roughness
+    and rain) run in an ensemble with a 512^2 one over a 600 s window, against
+    the FP32 oracle with its own CFL schedule: the same step count, depth
+    within 1e-5 m and momentum within 5e-5 m^2/s."""
+    import torch
+
+    from hybrid_flood_amd import synthetic as sy
+    from hybrid_flood_amd.ensemble import EnsembleRunner, member_from_case
+    from oracle import swe_oracle as o
+
+    t_end = 600.0
+    spec = sy.ensemble_members(64, 42)
+    k256 = next(k for k, s in enumerate(spec) if s[0] == 256)
+    k512 = next(k for k, s in enumerate(spec) if s[0] == 512)
+    members = []
+    for k in (k256, k512):
+        side, mseed, n_scale, rain_scale = spec[k]
+        case = sy.urban_case(side, dx=2.0, seed=mseed, duration_s=t_end)
+        members.append(member_from_case(case, device=cuda, n_scale=n_scale, rain_scale=rain_scale))
+    m = members[0]
+    p = m.params
+    op = o.OracleParams(bed=p.bed.cpu().numpy(), manning_n=p.manning_n.cpu().numpy(),
+                        domain_mask=p.domain_mask.cpu().numpy().astype(bool),
+                        rainfall_multiplier=p.rainfall_multiplier.cpu().numpy(),
+                        boundary_types=tuple(int(b) for b in p.boundary_types),
+                        rainfall_times_s=np.asarray(p.rainfall_times_s, np.float32),
+                        rainfall_rates_m_s=np.asarray(p.rainfall_rates_m_s, np.float32),
+                        dx=float(p.dx), cfl=float(p.cfl), dry_tolerance=float(p.dry_tolerance),
+                        max_dt=float(p.max_dt))
+    h0 = m.state.h.cpu().numpy()
+    z = np.zeros_like(h0)
+    ref = o.integrate_to_outputs(o.OracleState(h0, z, z), op, [0.0, t_end], max_steps=100_000)
+    ens = EnsembleRunner(members, variant="fast")
+    infos = ens.run(t_end, max_steps=100_000)
+    torch.cuda.synchronize()
+    assert infos[0]["steps_used"] == ref.steps_used
+    for f, atol in (("h", 1e-5), ("hu", 5e-5), ("hv", 5e-5)):
+        got = getattr(m.state, f).cpu().numpy()
+        assert np.abs(got - getattr(ref.states, f)[-1]).max() <= atol, f
+    assert infos[0]["rain_volume"] == pytest.approx(ref.rain_volume, rel=1e-9)
