@@ -305,12 +305,13 @@ __global__ __launch_bounds__(256) void k_ens_load(const Member* __restrict__ mem
   const size_t N = (size_t)G.nx * (size_t)G.ny;
   float* A = mb.buf[0];
   float inv = 0.0f;
-  double vol = 0.0;
+  double vol = 0.0, msum = 0.0;
   int nf = 0;
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < N;
        k += (size_t)gridDim.x * blockDim.x) {
     const float a0 = h[k], a1 = hu[k], a2 = hv[k];
     const int a = G.mask[k] != 0;
+    msum += a ? (double)G.mult[k] : 0.0;
     if (G.pk) {
       A[3 * k] = a0;
       A[3 * k + 1] = a1;
@@ -328,11 +329,15 @@ __global__ __launch_bounds__(256) void k_ens_load(const Member* __restrict__ mem
     vol += a ? (double)a0 : 0.0;
     nf |= !(isfinite(a0) && isfinite(a1) && isfinite(a2));
   }
-  const Totals tot = block_reduce(inv, vol, 0.0, 0.0, nf);
+  // The multiplier's FP64 sum over active cells rides in the clip partial
+  // (unused at the start): the member's fields may have changed in place
+  // since the last run, and its rain volume must follow them.
+  const Totals tot = block_reduce(inv, vol, msum, 0.0, nf);
   if (threadIdx.x == 0) {
     const int q = mb.wave_base + blockIdx.x;
     P.inv[q] = tot.inv;
     P.vol[q] = tot.vol;
+    P.clip[q] = tot.clip;
     P.nf[q] = tot.nf;
   }
 }
@@ -343,15 +348,16 @@ __global__ __launch_bounds__(256) void k_ens_init(const Member* __restrict__ mem
                                                   int parts, EnsPartials P) {
   const Member& mb = mem[m];
   float inv = 0.0f;
-  double vol = 0.0;
+  double vol = 0.0, msum = 0.0;
   int nf = 0;
   for (int k = threadIdx.x; k < parts; k += blockDim.x) {
     const int q = mb.wave_base + k;
     inv = fmaxf(inv, P.inv[q]);
     vol += P.vol[q];
+    msum += P.clip[q];
     nf |= P.nf[q];
   }
-  const Totals tot = block_reduce(inv, vol, 0.0, 0.0, nf);
+  const Totals tot = block_reduce(inv, vol, msum, 0.0, nf);
   const double t0 = mb.L.times[0];
   __shared__ int s_j;
   if (threadIdx.x == 0) s_j = 0;
@@ -360,6 +366,7 @@ __global__ __launch_bounds__(256) void k_ens_init(const Member* __restrict__ mem
     if (mb.L.rain_t[k] <= t0 && t0 < mb.L.rain_t[k + 1]) s_j = k;
   __syncthreads();
   if (threadIdx.x == 0) {
+    *const_cast<double*>(mb.L.sum_mult) = tot.clip;  // read by this run's finishes
     LoopControl* ctl = mb.ctl;
     ctl->t = t0;
     ctl->dt = 0.0;
@@ -492,25 +499,9 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
   }
   if (hipMemcpy(e->d_rain, rain_all.data(), sizeof(double) * rain_all.size(), hipMemcpyHostToDevice) != hipSuccess)
     return bail(efail(SWE_EHIP, "swe_ensemble_create: upload rain"));
-  // FP64 rain-multiplier sums over active cells, one member at a time, taken
-  // at creation (the rain volume of a member's diagnostics uses them).
-  std::vector<double> sums(n);
-  for (int m = 0; m < n; ++m) {
-    const SweParamsDesc& p = members[m].params;
-    const size_t N = (size_t)p.nx * p.ny;
-    std::vector<float> mult(N);
-    std::vector<uint8_t> mask(N);
-    if (hipMemcpy(mult.data(), p.rainfall_multiplier, 4 * N, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(mask.data(), p.domain_mask, N, hipMemcpyDeviceToHost) != hipSuccess)
-      return bail(efail(SWE_EHIP, "swe_ensemble_create: read multiplier"));
-    double s = 0.0;
-    for (size_t k = 0; k < N; ++k)
-      if (mask[k]) s += (double)mult[k];
-    sums[m] = s;
-    e->host[m].L.sum_mult = e->d_rain + rain_total + m;
-  }
-  if (hipMemcpy(e->d_rain + rain_total, sums.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
-    return bail(efail(SWE_EHIP, "swe_ensemble_create: upload multiplier sums"));
+  // Each member's FP64 rain-multiplier sum over active cells lives on the
+  // device; swe_ensemble_begin takes it from the member's current arrays.
+  for (int m = 0; m < n; ++m) e->host[m].L.sum_mult = e->d_rain + rain_total + m;
   int rc = layout(e);
   if (rc) return bail(rc);
   *out = e;

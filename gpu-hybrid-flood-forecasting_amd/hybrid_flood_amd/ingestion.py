@@ -698,26 +698,41 @@ def load_structured_grid(dem_path: str | Path, roughness_path: str | Path,
                              resolution_m, crs=domain.crs, transform=transform)
 
 
+# Columns of the rainfall table (jax_solver/shallow_water_2d.py:184-207): the
+# loader accepts a table only when it carries all of them.
+RAINFALL_COLUMNS = ("timestamp", "scenario", "rainfall_mm_hr", "is_missing_timestamp", "units")
+
+
 def load_rainfall_series(rainfall_path: str | Path, *, scenario: str,
                          default_rate_mm_hr: float = 0.0) -> tuple[np.ndarray, np.ndarray, float]:
     """Explicit mm/h rainfall of one scenario as (elapsed s, rates m/s, default m/s)
-    float32 arrays (jax_solver/shallow_water_2d.py:184-207)."""
-    import pandas as pd
+    (jax_solver/shallow_water_2d.py:184-207).
 
-    rainfall = pd.read_parquet(rainfall_path)
-    required = {"timestamp", "scenario", "rainfall_mm_hr", "is_missing_timestamp", "units"}
-    missing = required.difference(rainfall.columns)
-    if missing:
-        raise ValueError(f"Rainfall table is missing columns: {sorted(missing)}")
-    units = set(rainfall["units"].dropna().astype(str))
-    if units != {"mm/hr"}:
-        raise ValueError(f"Expected rainfall units 'mm/hr', found {sorted(units)}")
-    selected = rainfall.loc[rainfall["scenario"] == scenario].sort_values("timestamp")
-    if selected.empty:
+    The Parquet table must hold RAINFALL_COLUMNS, every non-null ``units``
+    entry must be "mm/hr", the scenario must have rows, and none of them may
+    be flagged missing or lack an intensity.  The scenario's rows are put in
+    time order; elapsed seconds count from the first one.  Times and rates
+    are float32, the default rate a Python float."""
+    import pyarrow.parquet as pq
+
+    absent = sorted(set(RAINFALL_COLUMNS) - set(pq.read_schema(rainfall_path).names))
+    if absent:
+        raise ValueError(f"Rainfall table is missing columns: {absent}")
+    table = pq.read_table(rainfall_path, columns=list(RAINFALL_COLUMNS))
+    column = {name: table.column(name).to_numpy(zero_copy_only=False) for name in RAINFALL_COLUMNS}
+    found = sorted({str(u) for u in column["units"] if u is not None})
+    if found != ["mm/hr"]:
+        raise ValueError(f"Expected rainfall units 'mm/hr', found {found}")
+    rows = np.flatnonzero(column["scenario"] == scenario)
+    if rows.size == 0:
         raise ValueError(f"Rainfall scenario {scenario!r} is unavailable.")
-    if selected["is_missing_timestamp"].any() or selected["rainfall_mm_hr"].isna().any():
+    intensity = np.asarray(column["rainfall_mm_hr"][rows], dtype=np.float64)
+    flagged = np.asarray(column["is_missing_timestamp"][rows], dtype=bool)
+    if flagged.any() or np.isnan(intensity).any():
         raise ValueError("Rainfall contains missing timestamps or intensities.")
-    stamps = pd.to_datetime(selected["timestamp"])
-    elapsed = (stamps - stamps.iloc[0]).dt.total_seconds().to_numpy(np.float32)
-    rates = selected["rainfall_mm_hr"].to_numpy(np.float32) * np.float32(MM_HR_TO_M_S)
-    return elapsed, rates.astype(np.float32), default_rate_mm_hr * MM_HR_TO_M_S
+    stamps = np.asarray(column["timestamp"][rows], dtype="datetime64[ns]")
+    order = np.argsort(stamps, kind="stable")
+    stamps, intensity = stamps[order], intensity[order]
+    seconds = (stamps - stamps[0]) / np.timedelta64(1, "s")
+    rates = intensity.astype(np.float32) * np.float32(MM_HR_TO_M_S)
+    return seconds.astype(np.float32), rates, default_rate_mm_hr * MM_HR_TO_M_S

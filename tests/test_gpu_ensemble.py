@@ -158,3 +158,39 @@ def test_config4_member_256_against_fp32_oracle(cuda):
         got = getattr(m.state, f).cpu().numpy()
         assert np.abs(got - getattr(ref.states, f)[-1]).max() <= atol, f
     assert infos[0]["rain_volume"] == pytest.approx(ref.rain_volume, rel=1e-9)
+
+
+def test_member_fields_changed_in_place_are_used(cuda):
+    """A member's rain multiplier scaled in place between two runs of the same
+    ensemble: the second run's FP64 rain volume follows the new field (the
+    member's multiplier sum is taken at every start), so its mass balance
+    still closes, and it equals a fresh ensemble built on the changed field."""
+    import torch
+
+    from hybrid_flood_amd.ensemble import EnsembleRunner
+
+    cases = [random_case(500 + k, s, open_sides=(0, 1, 0, 0), speed=0.6)
+             for k, s in enumerate([(40, 70), (33, 90)])]
+    for c in cases:  # a constant rate: the rain volume does not depend on the dt schedule
+        c["rain_t"] = np.array([0.0, 1000.0], np.float32)
+        c["rain_r"] = np.array([2e-4, 2e-4], np.float32)
+    members = [_member(c, cuda) for c in cases]
+    start = [[x.clone() for x in m.state] for m in members]
+    ens = EnsembleRunner(members, variant="fast")
+    first = ens.run(3.0, max_steps=5000)
+    for m, s0 in zip(members, start):
+        for dst, src in zip(m.state, s0):
+            dst.copy_(src)
+    members[1].params.rainfall_multiplier.mul_(0.5)
+    second = ens.run(3.0, max_steps=5000)
+    torch.cuda.synchronize()
+    assert second[1]["rain_volume"] == pytest.approx(0.5 * first[1]["rain_volume"], rel=1e-6)
+    assert second[0]["rain_volume"] == pytest.approx(first[0]["rain_volume"], rel=1e-12)
+    for info in second:
+        assert abs(info["mass_balance_rel"]) < 1e-6
+    cases[1]["rainfall_multiplier"] = members[1].params.rainfall_multiplier.cpu().numpy()
+    fresh_members = [_member(c, cuda) for c in cases]
+    fresh = EnsembleRunner(fresh_members, variant="fast").run(3.0, max_steps=5000)
+    assert fresh[1]["rain_volume"] == pytest.approx(second[1]["rain_volume"], rel=1e-12)
+    for a, b in zip(members[1].state, fresh_members[1].state):
+        assert torch.equal(a, b)

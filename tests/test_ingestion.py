@@ -418,6 +418,13 @@ def test_load_rainfall_series(tmp_path):
     df.drop(columns=["units"]).to_parquet(p)
     with pytest.raises(ValueError, match="missing columns"):
         ig.load_rainfall_series(p, scenario="wet")
+    for col, val in (("is_missing_timestamp", True), ("rainfall_mm_hr", float("nan"))):
+        df3 = df.copy()
+        df3.loc[2, col] = val
+        df3.to_parquet(p)
+        with pytest.raises(ValueError, match="missing timestamps or intensities"):
+            ig.load_rainfall_series(p, scenario="wet")
+        ig.load_rainfall_series(p, scenario="dry")  # the other scenario is unaffected
 
 
 def test_reference_module_paths_reexport_the_loaders():
