@@ -33,7 +33,8 @@ struct Grid {
   int nx, ny;
   int bt[4];  // left, right, bottom, top; 0 reflective, 1 open
   float g, half_g, dry, dx, dy, rdx, rdy;
-  float rho;  // dy / dx (the fast variant's CFL form)
+  float rho;   // dy / dx (the fast variant's CFL form)
+  float rhox;  // dx / dy (the fast variant's divergence form)
   const float* __restrict__ bed;
   const float* __restrict__ n;
   const uint8_t* __restrict__ mask;

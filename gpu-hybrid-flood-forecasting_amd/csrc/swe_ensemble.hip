@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
     StripAcc acc;
     if constexpr (M::kFast)
       march_strip_fast<true, PACKED>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
-                             local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dt_f,
+                             local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dtx_f,
                              ctl->dtrate_f, ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr,
                              nullptr, nullptr, acc);
     else
@@ -469,6 +469,7 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
     G.rdx = 1.0f / G.dx;
     G.rdy = 1.0f / G.dy;
     G.rho = (float)(p.dy / p.dx);
+    G.rhox = (float)(p.dx / p.dy);
     G.bed = p.bed;
     G.n = p.manning_n;
     G.mask = p.domain_mask;

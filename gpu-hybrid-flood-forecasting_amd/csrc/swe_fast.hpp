@@ -342,7 +342,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
                                                  const float* __restrict__ HV,
                                                  float* __restrict__ Ho, float* __restrict__ HUo,
                                                  float* __restrict__ HVo, int sx, int sy, int rows,
-                                                 float dt, float dtrate, float dtg, double dxd,
+                                                 float dtx, float dtrate, float dtg, double dxd,
                                                  double dyd, int rec_lo, int rec_hi,
                                                  float* __restrict__ cube_h,
                                                  float* __restrict__ cube_hu,
@@ -358,7 +358,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   const bool edge_x = LOOP && (sx == 0 || (sx + 1) * kStripCols >= G.nx);
   const int j0 = sy * rows;
   const int j1 = min(j0 + rows, G.ny);
-  const float rdx = G.rdx, rdy = G.rdy, dry = G.dry;
+  const float dry = G.dry;
   using IO = std::conditional_t<PACKED, PackedIO, StripIO>;
   IO io;
   if constexpr (PACKED)
@@ -480,12 +480,13 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     const float yn0 = SOUTH ? yc0 : yf0, yn1 = SOUTH ? yc1 : yf1, yn2 = SOUTH ? yc2 : yf2;  // north
     const float ys0 = SOUTH ? yf0 : yc0, ys1 = SOUTH ? yf1 : yc1, ys2 = SOUTH ? yf2 : yc2;  // south
 
-    const float d0 = fmaf(yn0 - ys0, rdy, (xl0 - xr0) * rdx);
-    const float d1 = fmaf(yn1 - ys1, rdy, (xl1 - xr1) * rdx);
-    const float d2 = fmaf(yn2 - ys2, rdy, (xl2 - xr2) * rdx);
-    const float hs = fmaf(dtrate, mult_j, fmaf(-dt, d0, cur.s.h));
-    const float hus = fmaf(-dt, d1, cur.hu);
-    const float hvs = fmaf(-dt, d2, cur.hv);
+    // dt ((xl - xr)/dx + (yn - ys)/dy) as (dt/dx) ((xl - xr) + (yn - ys) dx/dy)
+    const float d0 = fmaf(yn0 - ys0, G.rhox, xl0 - xr0);
+    const float d1 = fmaf(yn1 - ys1, G.rhox, xl1 - xr1);
+    const float d2 = fmaf(yn2 - ys2, G.rhox, xl2 - xr2);
+    const float hs = fmaf(dtrate, mult_j, fmaf(-dtx, d0, cur.s.h));
+    const float hus = fmaf(-dtx, d1, cur.hu);
+    const float hvs = fmaf(-dtx, d2, cur.hv);
     const float hn = fmaxf(hs, 0.0f);
     const float hd = fmaxf(hs, dry);  // = max(hn, dry): dry > 0
     const float rhd = __builtin_amdgcn_rcpf(hd);
@@ -599,7 +600,7 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
                                                  const float* __restrict__ HV,
                                                  float* __restrict__ Ho, float* __restrict__ HUo,
                                                  float* __restrict__ HVo, int sx, int sy, int rows,
-                                                 float dt, float dtrate, float dtg, double dxd,
+                                                 float dtx, float dtrate, float dtg, double dxd,
                                                  double dyd, int rec_lo, int rec_hi,
                                                  float* __restrict__ cube_h,
                                                  float* __restrict__ cube_hu,
@@ -607,11 +608,11 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
                                                  float mc = 0.0f) {
   static_assert(PACKED || !UNI, "the uniform-multiplier record is a packed layout");
   if (sy & 1)
-    march_strip_dir<LOOP, PACKED, true, UNI>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dt, dtrate,
+    march_strip_dir<LOOP, PACKED, true, UNI>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate,
                                              dtg, dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu,
                                              cube_hv, acc, mc);
   else
-    march_strip_dir<LOOP, PACKED, false, UNI>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dt, dtrate,
+    march_strip_dir<LOOP, PACKED, false, UNI>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate,
                                               dtg, dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu,
                                               cube_hv, acc, mc);
 }

@@ -45,6 +45,7 @@ struct StepPlan {
 // step kernel reads on entry.
 struct LoopControl {
   float dt_f, dtrate_f, dtg_f;  // float32(dt), float32(dt*rate), float32(dt*g)
+  float dtx_f;                  // float32(dt/dx), the fast variant's divergence scale
   int done;
   int rec_lo, rec_hi;  // output slots the pending step writes
   int ck_slot;         // checkpoint slot the pending step writes (-1: none)
@@ -132,6 +133,7 @@ __device__ void plan_next_step(LoopControl* ctl, const LoopConst& L, float m, do
   ctl->dt_f = (float)dt;
   ctl->dtrate_f = (float)(dt * rate);
   ctl->dtg_f = (float)(dt * L.gravity);
+  ctl->dtx_f = (float)(dt / L.dxd);
   ctl->rec_lo = oi;
   ctl->rec_hi = L.record ? hi : oi;
   const int after = ctl->steps + 1;
@@ -326,7 +328,7 @@ __device__ __forceinline__ unsigned cfl_bits(const unsigned long long* slots, in
 // numpy_solver/shallow_water_2d.py:274), so every workgroup gets the same dt.
 struct StepGo {
   double dt;
-  float dt_f, dtrate_f, dtg_f, m;
+  float dt_f, dtrate_f, dtg_f, dtx_f, m;
   int rec_lo, rec_hi, ck_slot;
   bool stop, nonfinite;
 };
@@ -337,7 +339,7 @@ __device__ __forceinline__ StepGo step_go(const StepPlan& pl, unsigned bits, con
   g.stop = pl.done != 0 || g.nonfinite;
   g.m = __uint_as_float(g.nonfinite ? 0u : bits);
   g.dt = 0.0;
-  g.dt_f = g.dtrate_f = g.dtg_f = 0.0f;
+  g.dt_f = g.dtrate_f = g.dtg_f = g.dtx_f = 0.0f;
   g.rec_lo = g.rec_hi = 0;
   g.ck_slot = -1;
   if (g.stop) return g;
@@ -364,6 +366,7 @@ __device__ __forceinline__ StepGo step_go(const StepPlan& pl, unsigned bits, con
   g.dt_f = (float)dt;
   g.dtrate_f = (float)(dt * pl.rate);
   g.dtg_f = (float)(dt * L.gravity);
+  g.dtx_f = (float)(dt / L.dxd);
   return g;
 }
 

@@ -45,24 +45,37 @@ template <class M, int MODE, bool PACKED = false>
 __global__ __launch_bounds__(kBlock) void k_step(
     Grid G, const float* __restrict__ H, const float* __restrict__ HU,
     const float* __restrict__ HV, float* __restrict__ Ho, float* __restrict__ HUo,
-    float* __restrict__ HVo, float dt_arg, float dtrate_arg, float dtg_arg, int rows,
+    float* __restrict__ HVo, float dt_arg, float dtx_arg, float dtrate_arg, float dtg_arg, int rows,
     int segs_x, int n_waves, int rev, int par, LoopControl* __restrict__ ctl, LoopConst L,
     Partials P, float* __restrict__ cube_h, float* __restrict__ cube_hu,
     float* __restrict__ cube_hv) {
-  float dt = dt_arg, dtrate = dtrate_arg, dtg = dtg_arg;
+  float dt = dt_arg, dtx = dtx_arg, dtrate = dtrate_arg, dtg = dtg_arg;
   int rec_lo = 0, rec_hi = 0;
   int steps_before = 0;  // fused loop: index of this step
   if constexpr (MODE == MODE_LOOP_FUSED) {
     // par: parity of this step.  Every workgroup derives the step from its
     // plan and the CFL maximum that the previous step left in the slots.
     if (ctl->done) return;
-    const StepPlan pl = ctl->plan[par];
-    const StepGo go = step_go(pl, cfl_bits(P.cfl_slots + kCflSlots * par, pl.steps), L);
-    steps_before = pl.steps;
+    // Wavefront 0 derives the step (float64 arithmetic every lane would
+    // repeat) and hands it to the others through LDS.
+    __shared__ StepGo s_go;
+    __shared__ int s_steps;
+    if (wave_in_block() == 0) {
+      const StepPlan pl = ctl->plan[par];
+      const StepGo g0 = step_go(pl, cfl_bits(P.cfl_slots + kCflSlots * par, pl.steps), L);
+      if (threadIdx.x == 0) {
+        s_go = g0;
+        s_steps = pl.steps;
+      }
+    }
+    __syncthreads();
+    const StepGo go = s_go;
+    steps_before = s_steps;
     if (go.stop) {
       // Every workgroup sees the same plan and slots, so all stop here;
       // workgroup 0 accounts the last step taken and marks the loop done.
       if (blockIdx.x == 0) {
+        const StepPlan pl = ctl->plan[par];
         if (pl.steps >= 1 && ctl->acc_steps < pl.steps)
           account_before<kFusedUnroll>(ctl, L, P, (1 - par) * P.stride, (int)gridDim.x, pl,
                                        go.m, go.nonfinite);
@@ -74,6 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
       return;
     }
     dt = go.dt_f;
+    dtx = go.dtx_f;
     dtrate = go.dtrate_f;
     dtg = go.dtg_f;
     rec_lo = go.rec_lo;
@@ -88,6 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
   } else if constexpr (MODE != MODE_STEP) {
     if (ctl->done) return;
     dt = ctl->dt_f;
+    dtx = ctl->dtx_f;
     dtrate = ctl->dtrate_f;
     dtg = ctl->dtg_f;
     rec_lo = ctl->rec_lo;
@@ -117,15 +132,15 @@ __global__ __launch_bounds__(kBlock) void k_step(
       // cell, L.sum_mult[2]; the packed static record then leaves it out.
       if (L.sum_mult[1] != 0.0)
         march_strip_fast<true, true, true>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
-                                           rows, dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
+                                           rows, dtx, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
                                            cube_h, cube_hu, cube_hv, acc, (float)L.sum_mult[2]);
       else
         march_strip_fast<true, true, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
-                                            rows, dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
+                                            rows, dtx, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
                                             cube_h, cube_hu, cube_hv, acc);
     } else if constexpr (M::kFast)
       march_strip_fast<MODE != MODE_STEP, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
-                                                 gw / segs_x, rows, dt, dtrate, dtg, L.dxd, L.dyd,
+                                                 gw / segs_x, rows, dtx, dtrate, dtg, L.dxd, L.dyd,
                                                  rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
     else
       march_strip<M, MODE != MODE_STEP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
@@ -498,6 +513,7 @@ struct SweContext {
   int alt_sweep = 1;         // odd device-loop steps sweep the strips in reverse order
   int sweep_rev = 0;         // reverse flag of the step launch being issued
   int step_par = 0;          // parity of the step launch being issued (fused loop)
+  float step_dtx = 0.0f;     // float32(dt/dx) of a caller-given step (swe_step, replays)
   double* dt_log = nullptr;  // caller's device buffer for the dt schedule
   int dt_log_cap = 0;
   float *ck_h = nullptr, *ck_hu = nullptr, *ck_hv = nullptr;  // caller's checkpoint cube
@@ -650,7 +666,7 @@ void launch_step(SweContext* c, const float* h, const float* hu, const float* hv
                  float* huo, float* hvo, float dt, float dtrate, float dtg, const LoopConst& L,
                  float* ch, float* chu, float* chv, hipStream_t s) {
   hipLaunchKernelGGL((k_step<M, MODE, PACKED>), dim3(c->step_blocks()), dim3(kBlock), c->step_lds, s, c->G, h, hu,
-                     hv, ho, huo, hvo, dt, dtrate, dtg, c->rows, c->segs_x(), c->n_waves(),
+                     hv, ho, huo, hvo, dt, c->step_dtx, dtrate, dtg, c->rows, c->segs_x(), c->n_waves(),
                      c->sweep_rev, c->step_par, c->d_ctl, L, c->P, ch, chu, chv);
 }
 
@@ -1007,6 +1023,7 @@ int swe_create(const SweParamsDesc* p, int32_t variant, SweContext** out) {
   G.rdx = 1.0f / G.dx;
   G.rdy = 1.0f / G.dy;
   G.rho = (float)(p->dy / p->dx);
+  G.rhox = (float)(p->dx / p->dy);
   G.bed = p->bed;
   G.n = p->manning_n;
   G.mask = p->domain_mask;
@@ -1171,6 +1188,7 @@ int swe_step(SweContext* c, SweState in, SweState out, double dt, double time_s,
   const double rate = host_interp(c, time_s);
   const float dt_f = (float)dt, dtrate_f = (float)(dt * rate), dtg_f = (float)(dt * c->gravity);
   const LoopConst L = c->loop_const(0, 0);
+  c->step_dtx = (float)(dt / c->dx);
   launch_step_any(c, MODE_STEP, in.h, in.hu, in.hv, out.h, out.hu, out.hv, dt_f, dtrate_f, dtg_f,
                   L, nullptr, nullptr, nullptr, (hipStream_t)stream);
   SWE_HIP_TRY(hipGetLastError());
@@ -1314,6 +1332,7 @@ int swe_replay(SweContext* c, SweState in, const double* dts, const double* ts, 
     float* dhu = c->buf(d, 1);
     float* dhv = c->buf(d, 2);
     const double rate = host_interp(c, ts[k]);
+    c->step_dtx = (float)(dts[k] / c->dx);
     launch_step_any(c, MODE_STEP, sh, shu, shv, dh, dhu, dhv, (float)dts[k],
                     (float)(dts[k] * rate), (float)(dts[k] * c->gravity), L, nullptr, nullptr,
                     nullptr, s);
@@ -1340,6 +1359,7 @@ int swe_replay_store(SweContext* c, SweState in, const double* dts, const double
     float* dhu = cube_hu + (size_t)k * N;
     float* dhv = cube_hv + (size_t)k * N;
     const double rate = host_interp(c, ts[k]);
+    c->step_dtx = (float)(dts[k] / c->dx);
     launch_step_any(c, MODE_STEP, sh, shu, shv, dh, dhu, dhv, (float)dts[k],
                     (float)(dts[k] * rate), (float)(dts[k] * c->gravity), L, nullptr, nullptr,
                     nullptr, s);
