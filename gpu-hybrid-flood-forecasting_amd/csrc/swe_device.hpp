@@ -39,10 +39,13 @@ struct Grid {
   const float* __restrict__ n;
   const uint8_t* __restrict__ mask;
   const float* __restrict__ mult;
-  // Packed static fields of the fast device loop, 3 floats per cell: bed,
-  // n^2 for an active cell or -1 for an inactive one, rain multiplier
-  // (null where unused).
+  // Packed static fields of the fast device loop: per cell (bed, n', rain
+  // multiplier), (bed, n') or the bed and a one-byte class (swe_fast.hpp), n'
+  // = n^2 for an active cell and -1 for an inactive one (null where unused).
   const float* __restrict__ pk;
+  // Class table of the class-indexed static record (256 floats: n^2 of each
+  // Manning value of the active cells; null where unused).
+  const float* __restrict__ cls_tab;
 };
 
 // ---------------------------------------------------------------------------

@@ -43,6 +43,14 @@ __device__ __forceinline__ bool wave_all(bool p) {
   return __builtin_amdgcn_ballot_w64(p) == __builtin_amdgcn_read_exec();
 }
 
+// Lane mask of the cells whose packed n' has its sign bit clear (active),
+// straight from the comparison: a ballot of a bool that lives across loop
+// iterations goes through a vector register (two extra vector instructions).
+__device__ __forceinline__ unsigned long long active_lanes(float np) {
+  constexpr int kSignedGreater = 38;
+  return __builtin_amdgcn_sicmp(__float_as_int(np), -1, kSignedGreater);
+}
+
 struct SideF {
   float h, eta, b, u, v;
   float p;  // hydrostatic pressure term g/2 h^2, as (g/2 h) h
@@ -163,6 +171,7 @@ struct RawCell {
   bool a;
   bool neg_hv;  // row beyond a reflective south/north edge
   float n, m;   // Manning n and rain multiplier (packed layout only)
+  unsigned c;   // kStaticCls: the cell's class, until settled
 };
 
 // Per-strip addressing through raw buffer resources.  Each array gets a
@@ -225,19 +234,50 @@ __device__ __forceinline__ StripIO strip_io(const Grid& G, const float* H, const
 // one 32-bit load standing for all three elements; the whole-vector form is
 // split back into 32-bit loads; the volatile form is made system-coherent,
 // which bypasses the L2 that neighbouring strips share.)
-// With a rain multiplier that has one value on every active cell (UNI: the
-// reference's default multiplier, the domain mask, and any scalar multiple of
-// it) the static record drops it, 8 bytes per cell (bed, n'), and the step
-// takes the value as a constant: the rain source is the same fused multiply-add
-// with the same operands, so the bits do not change.
+// Static layouts of the packed loop (LAY):
+//   kStatic12  (bed, n', multiplier), 12 bytes per cell;
+//   kStatic8   (bed, n'), 8 bytes, when the rain multiplier has one value on
+//              every active cell (the reference's default multiplier, the
+//              domain mask, and any scalar multiple of it), taken as a
+//              constant: the rain source is the same fused multiply-add with
+//              the same operands, so the bits do not change;
+//   kStaticCls bed (4 bytes) plus a one-byte class per cell, when moreover the
+//              Manning field takes at most 255 values on the active cells (a
+//              land-use roughness map): class c < 255 stands for n' = n^2 of
+//              that value, read from a table in LDS, and class 255 for an
+//              inactive cell (n' = -1).  The table holds the same float32
+//              products as the 8-byte record and the active flag is again
+//              the sign of n', so again the bits do not change.  Stored in
+//              blocks of 64 columns, 64 beds then 64 classes (320 bytes), so
+//              the static fields stay one stream; a row is
+//              ceil(nx / 64) blocks.
+enum { kStatic12 = 0, kStatic8 = 1, kStaticCls = 2 };
+constexpr unsigned kClsInactive = 255u;
+constexpr unsigned kClsBlockBytes = 320u;  // 64 beds + 64 classes
+
+__host__ __device__ __forceinline__ size_t cls_row_bytes(int nx) {
+  return (size_t)((nx + 63) / 64) * kClsBlockBytes;
+}
+
+// The class table of kStaticCls, one per workgroup (filled by the step
+// kernel before the march).
+__device__ __forceinline__ float* class_table() {
+  __shared__ float t[256];
+  return t;
+}
+
+// Per-strip addressing of the packed layout through raw buffer resources, as
+// StripIO: each array gets a resource based at the strip's lowest row, a row
+// is a wave-uniform byte offset (scalar register) and each lane adds its
+// fixed column offset, so no vector instruction runs per row for addressing.
 struct PackedIO {
-  const float* st;  // packed state
-  const float* sx;  // packed static fields
-  float* out;       // packed output state
-  unsigned off12;   // this lane's column, in bytes, in a packed state row
-  unsigned offsx;   // this lane's column, in bytes, in a packed static row
-  size_t row3;      // floats per packed state row (3 nx)
-  size_t rowsx;     // floats per packed static row (3 nx, or 2 nx with UNI)
+  Rsrc st, sx, out;  // packed state in, packed static fields, packed state out
+  unsigned off12;    // this lane's column, in bytes, in a packed state row
+  unsigned offsx;    // this lane's column, in bytes, in a packed static row
+  unsigned offcls;   // kStaticCls: this lane's class byte in a packed static row
+  unsigned row_st;   // bytes per packed state row (12 nx)
+  unsigned row_sx;   // bytes per packed static row (12 nx, 8 nx, or 320 ceil(nx / 64))
+  int jlo;           // row the resources start at
 };
 
 // A wave-uniform pointer, read from lane 0 so that it sits in scalar
@@ -251,56 +291,91 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
   return (T*)(((unsigned long long)hi << 32) | lo);
 }
 
-template <bool UNI>
-__device__ __forceinline__ PackedIO packed_io(const Grid& G, const float* S, float* O, int ic) {
+// Cache policy of the state's loads and stores: non-temporal (it streams
+// through once per step; measured 1-2 % faster at 4096^2 than plain ones).
+// The static fields keep plain loads.
+constexpr int kStreamNT = 2;
+
+// The multi-dword buffer loads return vectors; their elements are read as
+// .x/.y/.z of these types (a bit_cast of an indexed element of the builtin's
+// own return type has compiled to element 0 for every index).
+typedef unsigned U32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned U32x3 __attribute__((ext_vector_type(3)));
+
+template <int LAY>
+__device__ __forceinline__ PackedIO packed_io(const Grid& G, const float* S, float* O, int ic,
+                                              int jlo) {
   PackedIO io;
-  io.st = uniform_ptr(S);
-  io.sx = uniform_ptr(G.pk);
-  io.out = uniform_ptr(O);
+  io.jlo = jlo;
+  io.row_st = 12u * (unsigned)G.nx;
+  const size_t rows = (size_t)(G.ny - jlo);
+  const size_t st0 = (size_t)jlo * io.row_st;
+  io.st = make_rsrc(reinterpret_cast<const char*>(uniform_ptr(S)) + st0, rows * io.row_st);
+  io.out = make_rsrc(reinterpret_cast<char*>(uniform_ptr(O)) + st0, rows * io.row_st);
   io.off12 = (unsigned)ic * 12u;
-  io.offsx = (unsigned)ic * (UNI ? 8u : 12u);
-  io.row3 = 3 * (size_t)G.nx;
-  io.rowsx = (UNI ? 2 : 3) * (size_t)G.nx;
+  if constexpr (LAY == kStaticCls) {
+    const unsigned blk = (unsigned)ic / 64u * kClsBlockBytes, i = (unsigned)ic % 64u;
+    io.offsx = blk + 4u * i;
+    io.offcls = blk + 256u + i;
+    io.row_sx = (unsigned)cls_row_bytes(G.nx);
+  } else {
+    constexpr unsigned per = LAY == kStatic12 ? 12u : 8u;
+    io.offsx = (unsigned)ic * per;
+    io.offcls = 0;
+    io.row_sx = per * (unsigned)G.nx;
+  }
+  io.sx = make_rsrc(reinterpret_cast<const char*>(uniform_ptr(G.pk)) + (size_t)jlo * io.row_sx,
+                    rows * io.row_sx);
   return io;
-}
-
-// base, row3 and jj must be wave-uniform.  The row's address is pinned in
-// scalar registers (an empty asm statement, no instruction) so that the
-// compiler keeps the scalar-base + 32-bit lane-offset form of the global
-// load/store instead of folding the loop-invariant lane offset into a 64-bit
-// per-lane pointer that costs a vector add per row.
-using GlobalF = __attribute__((address_space(1))) float;
-
-__device__ __forceinline__ const GlobalF* packed_at(const float* base, size_t row3, int jj,
-                                                    unsigned off12) {
-  const __attribute__((address_space(1))) char* row =
-      (const __attribute__((address_space(1))) char*)(base + (size_t)jj * row3);
-  asm("" : "+s"(row));
-  asm("" : "+v"(off12));  // keeps the 32-bit offset visible at the access
-  return (const GlobalF*)(row + off12);
 }
 
 // Row j of the lane's column: clamped row index (numerics.py:29-55); the
 // reflective-edge sign flips are applied in side_of.  mc: the uniform
-// multiplier (UNI only).
-template <bool UNI>
+// multiplier (kStatic8 and kStaticCls).
+template <int LAY>
 __device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, int j, float mc) {
   const int jj = min(max(j, 0), G.ny - 1);
-  const GlobalF* st = packed_at(io.st, io.row3, jj, io.off12);
-  const GlobalF* sx = packed_at(io.sx, io.rowsx, jj, io.offsx);
+  const unsigned rs = (unsigned)(jj - io.jlo);
   RawCell r;
-  r.h = st[0];
-  r.hu = st[1];
-  r.hv = st[2];
-  r.b = sx[0];
-  r.n = sx[1];
-  r.m = UNI ? mc : sx[2];
-  r.a = !signbit(r.n);  // sign bit clear: active
+  const U32x3 st = __builtin_amdgcn_raw_buffer_load_b96(io.st, io.off12, rs * io.row_st, kStreamNT);
+  r.h = __uint_as_float(st.x);
+  r.hu = __uint_as_float(st.y);
+  r.hv = __uint_as_float(st.z);
+  const unsigned so = rs * io.row_sx;
+  if constexpr (LAY == kStatic12) {
+    const U32x3 sx = __builtin_amdgcn_raw_buffer_load_b96(io.sx, io.offsx, so, 0);
+    r.b = __uint_as_float(sx.x);
+    r.n = __uint_as_float(sx.y);
+    r.m = __uint_as_float(sx.z);
+  } else if constexpr (LAY == kStatic8) {
+    const U32x2 sx = __builtin_amdgcn_raw_buffer_load_b64(io.sx, io.offsx, so, 0);
+    r.b = __uint_as_float(sx.x);
+    r.n = __uint_as_float(sx.y);
+    r.m = mc;
+  } else {
+    r.b = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(io.sx, io.offsx, so, 0));
+    // the class is looked up when the row is used (settled), not here
+    r.c = __builtin_amdgcn_raw_buffer_load_b8(io.sx, io.offcls, so, 0);
+    r.m = mc;
+  }
+  if constexpr (LAY != kStaticCls) r.a = !signbit(r.n);  // sign bit clear: active
   r.neg_hv = (j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0);
   return r;
 }
 
-template <bool UNI>
+// The row's class looked up: n' and the active flag as the 8-byte record
+// gives them.  Called where the row is first used, so the loads of the rows
+// behind it stay in flight.
+template <int LAY>
+__device__ __forceinline__ RawCell settled(RawCell r) {
+  if constexpr (LAY == kStaticCls) {
+    r.n = class_table()[r.c];
+    r.a = !signbit(r.n);
+  }
+  return r;
+}
+
+template <int LAY>
 __device__ __forceinline__ RawCell load_raw(const Grid& G, const StripIO& io, int j, float) {
   const int jj = min(max(j, 0), G.ny - 1);
   RawCell r;
@@ -331,12 +406,28 @@ __device__ __forceinline__ RowCell side_of(const RawCell& r, bool neg_hu, float 
   return c;
 }
 
+// Rows whose loads are in flight ahead of the current row of a march.  At
+// 4096^2: 2 rows (78 registers, 6 wavefronts per SIMD) ran 1-2 % slower than
+// 3 (85 registers, 5 per SIMD); 4 and 5 no faster than 3.
+constexpr int kRowsAhead = 3;
+
+// f(integral_constant<int, U>) for U = B .. E-1, in order: indices derived
+// from U are compile-time constants in the source, so arrays indexed by them
+// stay in registers.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 // One wavefront's strip: columns sx*62-1 .. sx*62+62, rows sy*rows ..
 // sy*rows+rows-1.  sx, sy and rows must be wave-uniform.  The raw inputs of
 // row j+2 are in flight while row j is computed.  PACKED: H and Ho are packed
 // states (3 floats per cell, HU/HV unused) and the static fields come from
 // G.pk; the arithmetic is the same, so both layouts give identical bits.
-template <bool LOOP, bool PACKED, bool SOUTH, bool UNI>
+template <bool LOOP, bool PACKED, bool SOUTH, int LAY>
 __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
                                                  const float* __restrict__ HV,
@@ -362,7 +453,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   using IO = std::conditional_t<PACKED, PackedIO, StripIO>;
   IO io;
   if constexpr (PACKED)
-    io = packed_io<UNI>(G, H, Ho, ic);
+    io = packed_io<LAY>(G, H, Ho, ic, max(j0 - 1, 0));
   else
     io = strip_io(G, H, HU, HV, max(j0 - 1, 0), ic);
   const size_t out0 = (size_t)j0 * (size_t)G.nx, out_cells = (size_t)(j1 - j0) * (size_t)G.nx;
@@ -377,21 +468,27 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   // (j1-1 .. j0); dj is the step.  "Ahead" below means the march direction.
   constexpr int dj = SOUTH ? -1 : 1;
   const int jf = SOUTH ? j1 - 1 : j0;  // first row of the march
-  // Prologue: the face behind row jf, rows jf (current), jf+dj and jf+2dj
-  // (in flight).
-  RowCell cur = side_of(load_raw<UNI>(G, io, jf - dj, mc), neg_hu, dry, G.half_g);
-  RawCell next = load_raw<UNI>(G, io, jf, mc);
-  RawCell r2;
+  // Raw rows rotate through kRowsAhead + 1 register sets (raw[]): a row is
+  // loaded kRowsAhead rows before it becomes the current row.  Prologue: the
+  // face behind row jf, row jf (current) and rows jf+dj .. jf+(kRowsAhead-1)dj
+  // (in flight, raw[0 ..]).
+  constexpr int kSets = kRowsAhead + 1;
+  RawCell raw[kSets];
+  RowCell cur = side_of(settled<LAY>(load_raw<LAY>(G, io, jf - dj, mc)), neg_hu, dry, G.half_g);
   float yc0, yc1, yc2;  // carried y face: between the current row and the one behind it
   float nn, mult;
   {
-    const RowCell c0 = side_of(next, neg_hu, dry, G.half_g);
+    const RawCell first = settled<LAY>(load_raw<LAY>(G, io, jf, mc));
+    const RowCell c0 = side_of(first, neg_hu, dry, G.half_g);
     if constexpr (PACKED) {
-      nn = next.n;
-      mult = next.m;
+      nn = first.n;
+      mult = first.m;
     }
-    next = load_raw<UNI>(G, io, jf + dj, mc);
-    r2 = load_raw<UNI>(G, io, SOUTH ? max(jf - 2, j0 - 1) : min(jf + 2, j1), mc);
+    raw[0] = load_raw<LAY>(G, io, jf + dj, mc);
+    static_for<2, kRowsAhead>([&](auto mc_) __attribute__((always_inline)) {
+      constexpr int m = decltype(mc_)::value;
+      raw[m - 1] = load_raw<LAY>(G, io, SOUTH ? max(jf - m, j0 - 1) : min(jf + m, j1), mc);
+    });
     if constexpr (!SOUTH) {  // south face of row j0
       const FaceFlux fy = face_fast_any<false>(G, cur.s, c0.s, wave_all(cur.s.a && c0.s.a));
       if (LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
@@ -415,17 +512,18 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   float clip32 = 0.0f;  // clip depth is small; float32 per lane over the strip
 
   // Row j: `cur` is row j, `north` row j+dj (the next row of the march,
-  // loaded two rows ago; row j+2dj is in flight too).  Issues the loads of
-  // row j+3dj into `ahead`, takes row j+dj's n/multiplier into
+  // loaded kRowsAhead - 1 rows ago; the rows up to j+(kRowsAhead-1)dj are in
+  // flight too).  Issues the loads of row j+kRowsAhead dj into `ahead`, takes row j+dj's n/multiplier into
   // nn_next/mult_next, and leaves row j+dj's side in `up`.
   auto row = [&](int j, const RowCell& cur, RowCell& up, const RawCell& north, RawCell& ahead,
                  float nn_j, float mult_j, float& nn_next, float& mult_next) {
     // Unconditional, so the wait counts stay exact; past the strip the
     // clamped rows were just loaded and come from the cache.
-    ahead = load_raw<UNI>(G, io, SOUTH ? max(j - 3, j0 - 1) : min(j + 3, j1), mc);
+    ahead = load_raw<LAY>(G, io, SOUTH ? max(j - kRowsAhead, j0 - 1) : min(j + kRowsAhead, j1), mc);
+    const RawCell nrow = settled<LAY>(north);
     if constexpr (PACKED) {  // row j+dj's fields arrived with its state
-      nn_next = north.n;
-      mult_next = north.m;
+      nn_next = nrow.n;
+      mult_next = nrow.m;
     } else {
       const int jn = SOUTH ? max(j - 1, j0) : min(j + 1, j1 - 1);
       nn_next = io.ld(io.n, jn);
@@ -463,12 +561,17 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
       if (i == G.nx) acc.bnd -= (double)fx.f0 * dyd;
     }
     // The y face ahead of row j; a y face's A side is always its south cell.
-    up = side_of(north, neg_hu, dry, G.half_g);
+    up = side_of(nrow, neg_hu, dry, G.half_g);
     const SideF& ys = SOUTH ? up.s : cur.s;  // south cell of the face
     const SideF& yn = SOUTH ? cur.s : up.s;  // north cell
     const bool as = SOUTH ? up.s.a : ca, an = SOUTH ? ca : up.s.a;
     FaceFlux gy;
-    if (wave_all(ca && up.s.a))
+    bool all_y;  // both cells of every lane's y face are active
+    if constexpr (PACKED)
+      all_y = (active_lanes(nn_j) & active_lanes(nrow.n)) == __builtin_amdgcn_read_exec();
+    else
+      all_y = wave_all(ca && up.s.a);
+    if (all_y)
       gy = face_fast<false, false>(G, ys.h, ys.eta, ys.b, ys.v, ys.u, ys.p, true, yn.h, yn.eta,
                                    yn.b, yn.v, yn.u, yn.p, true);
     else
@@ -488,7 +591,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     const float hus = fmaf(-dtx, d1, cur.hu);
     const float hvs = fmaf(-dtx, d2, cur.hv);
     const float hn = fmaxf(hs, 0.0f);
-    const float hd = fmaxf(hs, dry);  // = max(hn, dry): dry > 0
+    const float hd = max_nc(hs, dry);  // = max(hn, dry): dry > 0
     const float rhd = __builtin_amdgcn_rcpf(hd);
     const float q = __builtin_amdgcn_sqrtf(fmaf(hus, hus, hvs * hvs));
     // K = dtg n^2 q / hd^(7/3), the power as 2^(-7/3 log2 hd) on the hardware log/exp
@@ -502,10 +605,9 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     const float hv_o = wet ? hvs * rD : 0.0f;
     if (own) {
       if constexpr (PACKED) {
-        GlobalF* o = const_cast<GlobalF*>(packed_at(io.out, io.row3, j, io.off12));
-        o[0] = h_o;
-        o[1] = hu_o;
-        o[2] = hv_o;
+        const U32x3 v = {__float_as_uint(h_o), __float_as_uint(hu_o), __float_as_uint(hv_o)};
+        __builtin_amdgcn_raw_buffer_store_b96(v, io.out, io.off12,
+                                              (unsigned)(j - io.jlo) * io.row_st, kStreamNT);
       } else {
         const unsigned so = (unsigned)(j - j0) * io.pitch;
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, h_o), oh, io.off4, so,
@@ -550,29 +652,39 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     yc2 = gy.fn + (SOUTH ? gy.cA : gy.cB);
   };
 
-  // Four rows per trip with the raw rows rotating through four sets of
-  // registers: a row's raw inputs stay live over four rows (loaded three
-  // rows ahead, read as the next row, then as the current row), so fewer
-  // sets would make the compiler copy one at every loop edge, and wait there
-  // for the loads in flight.  The derived row values alternate between two.
-  // The raw sets, by the row (relative to the trip's first row j) they hold
-  // when row j starts: next = j+1, r2 = j+2 (both from the prologue or the
-  // previous trip), r3 and r1 are filled with rows j+3 and j+4 on the way.
-  RowCell c1;
-  RawCell r1, r3;
-  float nn1 = 0.0f, mult1 = 0.0f;
+  // kSets rows per trip, so every raw set keeps its register slot: a row's
+  // raw inputs stay live over kSets rows (loaded, read as the next row, then
+  // as the current row), and fewer sets would make the compiler copy one at
+  // every loop edge, and wait there for the loads in flight.  Row j+u of a
+  // trip reads row j+u+1 from raw[u % kSets] and loads row j+u+kRowsAhead into
+  // raw[(u + kRowsAhead - 1) % kSets].  The derived row values alternate
+  // between two sets and are handed back after an odd-length trip; they are
+  // computed values, so that copy waits for nothing.  All indices are
+  // compile-time constants after unrolling, so raw[] stays in registers.
+  RowCell rc[2] = {cur, cur};
+  float nnv[2] = {nn, 0.0f}, multv[2] = {mult, 0.0f};
   const int n = j1 - j0;
   int k = 0;
-  for (; k + 3 < n; k += 4) {
+  for (; k + kSets - 1 < n; k += kSets) {
     const int j = jf + k * dj;
-    row(j, cur, c1, next, r3, nn, mult, nn1, mult1);
-    row(j + dj, c1, cur, r2, r1, nn1, mult1, nn, mult);
-    row(j + 2 * dj, cur, c1, r3, next, nn, mult, nn1, mult1);
-    row(j + 3 * dj, c1, cur, r1, r2, nn1, mult1, nn, mult);
+    static_for<0, kSets>([&](auto uc) __attribute__((always_inline)) {
+      constexpr int u = decltype(uc)::value;
+      row(j + u * dj, rc[u & 1], rc[(u + 1) & 1], raw[u % kSets], raw[(u + kRowsAhead - 1) % kSets],
+          nnv[u & 1], multv[u & 1], nnv[(u + 1) & 1], multv[(u + 1) & 1]);
+    });
+    if constexpr (kSets & 1) {
+      rc[0] = rc[1];
+      nnv[0] = nnv[1];
+      multv[0] = multv[1];
+    }
   }
-  if (k < n) row(jf + k * dj, cur, c1, next, r3, nn, mult, nn1, mult1);
-  if (k + 1 < n) row(jf + (k + 1) * dj, c1, cur, r2, r1, nn1, mult1, nn, mult);
-  if (k + 2 < n) row(jf + (k + 2) * dj, cur, c1, r3, next, nn, mult, nn1, mult1);
+  static_for<0, kSets - 1>([&](auto uc) __attribute__((always_inline)) {
+    constexpr int u = decltype(uc)::value;
+    if (k + u < n)
+      row(jf + (k + u) * dj, rc[u & 1], rc[(u + 1) & 1], raw[u % kSets],
+          raw[(u + kRowsAhead - 1) % kSets], nnv[u & 1], multv[u & 1], nnv[(u + 1) & 1],
+          multv[(u + 1) & 1]);
+  });
   // yc0 now holds the mass flux through the last face of the march: the north
   // face of row j1-1, or (SOUTH) the south face of row j0.
   if constexpr (!SOUTH) {
@@ -594,7 +706,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
 // the second read is an L2 hit.  The direction does not change any cell's
 // arithmetic: every face takes the same operands, and a cell combines its
 // faces in the same order (north minus south, east minus west).
-template <bool LOOP, bool PACKED = false, bool UNI = false>
+template <bool LOOP, bool PACKED = false, int LAY = kStatic12>
 __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
                                                  const float* __restrict__ HV,
@@ -606,13 +718,13 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
                                                  float* __restrict__ cube_hu,
                                                  float* __restrict__ cube_hv, StripAcc& acc,
                                                  float mc = 0.0f) {
-  static_assert(PACKED || !UNI, "the uniform-multiplier record is a packed layout");
+  static_assert(PACKED || LAY == kStatic12, "the compact static records are packed layouts");
   if (sy & 1)
-    march_strip_dir<LOOP, PACKED, true, UNI>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate,
+    march_strip_dir<LOOP, PACKED, true, LAY>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate,
                                              dtg, dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu,
                                              cube_hv, acc, mc);
   else
-    march_strip_dir<LOOP, PACKED, false, UNI>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate,
+    march_strip_dir<LOOP, PACKED, false, LAY>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate,
                                               dtg, dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu,
                                               cube_hv, acc, mc);
 }

@@ -153,7 +153,7 @@ struct Totals {
 };
 
 template <int WAVES = kWavesPerBlock>
-__device__ Totals block_reduce(float inv, double vol, double clip, double bnd, int nf) {
+__device__ __forceinline__ Totals block_reduce(float inv, double vol, double clip, double bnd, int nf) {
   __shared__ float s_inv[WAVES];
   __shared__ double s_vol[WAVES], s_clip[WAVES], s_bnd[WAVES];
   __shared__ int s_nf[WAVES];

@@ -52,6 +52,13 @@ __global__ __launch_bounds__(kBlock) void k_step(
   float dt = dt_arg, dtx = dtx_arg, dtrate = dtrate_arg, dtg = dtg_arg;
   int rec_lo = 0, rec_hi = 0;
   int steps_before = 0;  // fused loop: index of this step
+  int lay = kStatic12;   // packed loop: static record (L.sum_mult[3], set at the loop start)
+  if constexpr (PACKED) {
+    static_assert(kBlock == 256, "one class-table entry per thread");
+    lay = (int)L.sum_mult[3];
+    if (lay == kStaticCls) class_table()[threadIdx.x] = G.cls_tab[threadIdx.x];
+    if constexpr (MODE != MODE_LOOP_FUSED) __syncthreads();  // fused: the entry's barrier below
+  }
   if constexpr (MODE == MODE_LOOP_FUSED) {
     // par: parity of this step.  Every workgroup derives the step from its
     // plan and the CFL maximum that the previous step left in the slots.
@@ -128,16 +135,22 @@ __global__ __launch_bounds__(kBlock) void k_step(
   StripAcc acc;
   if (gw < n_waves) {
     if constexpr (M::kFast && PACKED) {
-      // L.sum_mult[1]: the rain multiplier has one value on every active
-      // cell, L.sum_mult[2]; the packed static record then leaves it out.
-      if (L.sum_mult[1] != 0.0)
-        march_strip_fast<true, true, true>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
-                                           rows, dtx, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
-                                           cube_h, cube_hu, cube_hv, acc, (float)L.sum_mult[2]);
+      // The static record chosen at the loop start; the uniform multiplier
+      // is L.sum_mult[2].
+      const float mc = (float)L.sum_mult[2];
+      if (lay == kStaticCls)
+        march_strip_fast<true, true, kStaticCls>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
+                                                 gw / segs_x, rows, dtx, dtrate, dtg, L.dxd,
+                                                 L.dyd, rec_lo, rec_hi, cube_h, cube_hu, cube_hv,
+                                                 acc, mc);
+      else if (lay == kStatic8)
+        march_strip_fast<true, true, kStatic8>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
+                                               gw / segs_x, rows, dtx, dtrate, dtg, L.dxd, L.dyd,
+                                               rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc, mc);
       else
-        march_strip_fast<true, true, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
-                                            rows, dtx, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
-                                            cube_h, cube_hu, cube_hv, acc);
+        march_strip_fast<true, true, kStatic12>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
+                                                gw / segs_x, rows, dtx, dtrate, dtg, L.dxd, L.dyd,
+                                                rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
     } else if constexpr (M::kFast)
       march_strip_fast<MODE != MODE_STEP, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
                                                  gw / segs_x, rows, dtx, dtrate, dtg, L.dxd, L.dyd,
@@ -351,7 +364,8 @@ __global__ __launch_bounds__(kBlock) void k_sum_mult(Grid G, double* __restrict_
 // One workgroup: out[0] = the sum of sum_part[0 .. n) in a fixed order;
 // out[1] = 1 if the multiplier has one bit pattern on every active cell (or
 // there is no active cell), else 0; out[2] = that value (0 without active
-// cells).
+// cells); out[3] = the packed loop's static record without classes
+// (kStatic8 for a uniform multiplier, else kStatic12).
 __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ sum_part,
                                                          const unsigned* __restrict__ bits_part,
                                                          int n, double* __restrict__ out) {
@@ -375,6 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restric
     out[0] = t.vol;
     out[1] = (none || lo == hi) ? 1.0 : 0.0;
     out[2] = none ? 0.0 : (double)__uint_as_float(lo);
+    out[3] = (none || lo == hi) ? (double)kStatic8 : (double)kStatic12;
   }
 }
 
@@ -390,17 +405,120 @@ __global__ __launch_bounds__(kBlock) void k_fill3(float* __restrict__ a, float* 
   }
 }
 
-// Packed layout of the fast device loop (see swe_fast.hpp): static fields and
-// state in, state out.  The static record is (bed, n') when the multiplier is
-// uniform over the active cells (sm[1], from k_sum_partials), else (bed, n',
-// multiplier).
-__global__ __launch_bounds__(kBlock) void k_pack_static(Grid G, float* __restrict__ pk, size_t N,
-                                                        const double* __restrict__ sm) {
+// ---- Class-indexed static record (swe_fast.hpp, kStaticCls) ---------------
+// At a loop start the Manning values of the active cells are gathered into a
+// 512-slot open-addressing table of their bit patterns; with at most 255 of
+// them (and a uniform rain multiplier) every active cell is stored as its
+// value's class and the step reads n^2 from a 255-entry table.
+constexpr int kClsSlots = 512;
+
+struct ClassBuf {
+  unsigned long long* keys;  // kClsSlots: 0 = empty, else (1 << 32) | float bits of n
+  float* vals;               // 256: n^2 of class c
+  unsigned char* slot_cls;   // kClsSlots: class of the value in slot s
+  int* overflow;             // the table filled up
+};
+
+__device__ __forceinline__ unsigned cls_hash(unsigned long long key) {
+  return (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 55);  // 9 bits: kClsSlots
+}
+
+__device__ __forceinline__ unsigned long long cls_key(float n) {
+  return (1ull << 32) | (unsigned long long)__float_as_uint(n);
+}
+
+__global__ __launch_bounds__(kClsSlots) void k_cls_clear(ClassBuf B) {
+  B.keys[threadIdx.x] = 0ull;
+  if (threadIdx.x == 0) *B.overflow = 0;
+}
+
+// Insert the Manning value of every active cell (the first of each run of
+// equal values along a wavefront: cell k - 1 is the west lane's cell).
+__global__ __launch_bounds__(kBlock) void k_cls_insert(Grid G, ClassBuf B) {
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock) {
+    const int a = G.mask[k] != 0;
+    const unsigned bits = __float_as_uint(G.n[k]);
+    const int wa = from_west(a);
+    const unsigned wbits = (unsigned)from_west((int)bits);
+    if (!a || (lane > 0 && wa && wbits == bits)) continue;
+    const unsigned long long key = (1ull << 32) | bits;
+    unsigned h = cls_hash(key);
+    bool placed = false;
+    for (int p = 0; p < kClsSlots && !placed; ++p) {
+      // A plain read first: after its first insert a value is found without
+      // an atomic (a few values over millions of cells would otherwise
+      // serialise on their slots); a stale empty slot falls through to the
+      // compare-and-swap, which sees the current key.
+      unsigned long long old = B.keys[h];
+      if (old == 0ull) old = atomicCAS(B.keys + h, 0ull, key);
+      placed = old == 0ull || old == key;
+      h = (h + 1) & (kClsSlots - 1);
+    }
+    if (!placed) atomicOr(B.overflow, 1);
+  }
+}
+
+// One workgroup of kClsSlots threads: number the used slots in slot order,
+// fill the class table, and record the static layout of this loop in sm[3]
+// (kStaticCls with at most 255 classes, a uniform multiplier and `allow`;
+// else kStatic8 with a uniform multiplier; else kStatic12).
+__global__ __launch_bounds__(kClsSlots) void k_cls_finish(ClassBuf B, double* __restrict__ sm,
+                                                          int allow) {
+  __shared__ int s_scan[kClsSlots];
+  const int t = threadIdx.x;
+  const unsigned long long key = B.keys[t];
+  const int used = key != 0ull;
+  s_scan[t] = used;
+  __syncthreads();
+  for (int d = 1; d < kClsSlots; d <<= 1) {  // inclusive scan
+    const int v = t >= d ? s_scan[t - d] : 0;
+    __syncthreads();
+    s_scan[t] += v;
+    __syncthreads();
+  }
+  const int count = s_scan[kClsSlots - 1];
   const bool uni = sm[1] != 0.0;
+  const bool ok = allow && uni && *B.overflow == 0 && count <= (int)kClsInactive;
+  if (t < 256) B.vals[t] = t == (int)kClsInactive ? -1.0f : 0.0f;  // inactive: n' = -1
+  __syncthreads();
+  if (ok && used) {
+    const int c = s_scan[t] - 1;
+    const float n = __uint_as_float((unsigned)key);
+    B.vals[c] = n * n;  // as the 8-byte record's n'
+    B.slot_cls[t] = (unsigned char)c;
+  }
+  if (t == 0) sm[3] = ok ? (double)kStaticCls : (uni ? (double)kStatic8 : (double)kStatic12);
+}
+
+// Packed layout of the fast device loop (see swe_fast.hpp): static fields and
+// state in, state out.  The static record is the one k_cls_finish chose in
+// sm[3]: the bed and the class (5 bytes, in blocks of 64 columns), (bed, n')
+// with a uniform multiplier, else (bed, n', multiplier).
+__global__ __launch_bounds__(kBlock) void k_pack_static(Grid G, float* __restrict__ pk, size_t N,
+                                                        const double* __restrict__ sm,
+                                                        ClassBuf B) {
+  const int lay = (int)sm[3];
+  const size_t pitch = cls_row_bytes(G.nx);
   for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
        k += (size_t)gridDim.x * kBlock) {
     const float b = G.bed[k], n2 = G.mask[k] ? G.n[k] * G.n[k] : -1.0f;
-    if (uni) {
+    if (lay == kStaticCls) {
+      unsigned c = kClsInactive;
+      if (G.mask[k]) {
+        const unsigned long long key = cls_key(G.n[k]);
+        unsigned h = cls_hash(key);
+        int p = 0;  // the key is in the table (k_cls_insert); the bound only ends the search
+        for (; p < kClsSlots && B.keys[h] != key; ++p) h = (h + 1) & (kClsSlots - 1);
+        if (p < kClsSlots) c = B.slot_cls[h];
+      }
+      const size_t j = k / (size_t)G.nx, i = k % (size_t)G.nx;
+      unsigned char* blk = reinterpret_cast<unsigned char*>(pk) + j * pitch + i / 64 * kClsBlockBytes;
+      reinterpret_cast<float*>(blk)[i % 64] = b;
+      blk[256 + i % 64] = (unsigned char)c;
+    } else if (lay == kStatic8) {
       pk[2 * k] = b;
       pk[2 * k + 1] = n2;
     } else {
@@ -492,7 +610,7 @@ struct SweContext {
   double dx = 1, dy = 1, gravity = 9.81, cfl = 0.45, dry = 1e-4, max_dt = 10, rain_default = 0;
   std::vector<double> rain_t, rain_r;
   // [FP64 rain-multiplier sum over active cells, 1 if the multiplier is
-  // uniform there, its value, unused]
+  // uniform there, its value, static record of the packed loop]
   double* d_sum_mult = nullptr;
   double* d_rain = nullptr;  // [rain_t | rain_r]
   double* d_times = nullptr;
@@ -502,7 +620,10 @@ struct SweContext {
   Partials P{};
   int parts_cap = 0;
   float* work = nullptr;  // 2 x 3 x N ping-pong state (SoA, or packed: see ws())
-  float* pk = nullptr;    // packed static fields of the fast device loop (3 x N)
+  float* pk = nullptr;    // packed static fields of the fast device loop (largest record)
+  void* cls_mem = nullptr;   // the Manning class table of the packed loop (ClassBuf)
+  ClassBuf cls{};
+  int static_classes = 1;    // the packed loop may store the Manning value as a class
   bool loop_packed = false;  // layout of the work state of the current device loop
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t graph = nullptr;
@@ -614,8 +735,19 @@ int copy_state(SweContext* c, float* dh, float* dhu, float* dhv, const float* sh
 int ensure_work(SweContext* c) {
   if (!c->work) SWE_HIP_TRY(hipMalloc(&c->work, sizeof(float) * 6 * c->cells()));
   if (!c->pk) {
-    SWE_HIP_TRY(hipMalloc(&c->pk, sizeof(float) * 3 * c->cells()));
+    // the largest static record: 3 floats per cell, or the class blocks
+    const size_t bytes = std::max(sizeof(float) * 3 * c->cells(), cls_row_bytes(c->G.nx) * (size_t)c->G.ny);
+    SWE_HIP_TRY(hipMalloc(&c->pk, bytes));
     c->G.pk = c->pk;
+  }
+  if (!c->cls_mem) {
+    constexpr size_t kKeys = sizeof(unsigned long long) * kClsSlots, kVals = sizeof(float) * 256;
+    SWE_HIP_TRY(hipMalloc(&c->cls_mem, kKeys + kVals + kClsSlots + sizeof(int)));
+    char* m = static_cast<char*>(c->cls_mem);
+    c->cls = ClassBuf{reinterpret_cast<unsigned long long*>(m), reinterpret_cast<float*>(m + kKeys),
+                      reinterpret_cast<unsigned char*>(m + kKeys + kVals),
+                      reinterpret_cast<int*>(m + kKeys + kVals + kClsSlots)};
+    c->G.cls_tab = c->cls.vals;
   }
   return SWE_OK;
 }
@@ -643,7 +775,12 @@ int stage_in(SweContext* c, const SweState& src, hipStream_t s) {
   if (!c->loop_packed) return copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), src.h, src.hu, src.hv, s);
   const size_t N = c->cells();
   const int nb = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 8192);
-  hipLaunchKernelGGL(k_pack_static, dim3(nb), dim3(kBlock), 0, s, c->G, c->pk, N, c->d_sum_mult);
+  hipLaunchKernelGGL(k_cls_clear, dim3(1), dim3(kClsSlots), 0, s, c->cls);
+  hipLaunchKernelGGL(k_cls_insert, dim3(nb), dim3(kBlock), 0, s, c->G, c->cls);
+  hipLaunchKernelGGL(k_cls_finish, dim3(1), dim3(kClsSlots), 0, s, c->cls, c->d_sum_mult,
+                     c->static_classes);
+  hipLaunchKernelGGL(k_pack_static, dim3(nb), dim3(kBlock), 0, s, c->G, c->pk, N, c->d_sum_mult,
+                     c->cls);
   hipLaunchKernelGGL(k_pack_state, dim3(nb), dim3(kBlock), 0, s, src.h, src.hu, src.hv, c->ws(0), N);
   SWE_HIP_TRY(hipGetLastError());
   return SWE_OK;
@@ -1068,6 +1205,7 @@ int swe_destroy(SweContext* c) {
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->work) (void)hipFree(c->work);
   if (c->pk) (void)hipFree(c->pk);
+  if (c->cls_mem) (void)hipFree(c->cls_mem);
   if (c->P.cfl_slots) (void)hipFree(c->P.cfl_slots);
   if (c->d_times) (void)hipFree(c->d_times);
   if (c->d_ctl) (void)hipFree(c->d_ctl);
@@ -1137,6 +1275,23 @@ int swe_set_loop_finish(SweContext* c, int32_t split) {
     c->graph = nullptr;
     c->graph_key.clear();
   }
+  return SWE_OK;
+}
+
+int swe_set_static_classes(SweContext* c, int32_t enable) {
+  if (!c || (enable != 0 && enable != 1))
+    return fail(SWE_EINVAL, "swe_set_static_classes: enable must be 0 or 1");
+  if (c->loop_active) return fail(SWE_EINVAL, "swe_set_static_classes: a device loop is active");
+  c->static_classes = enable;
+  return SWE_OK;
+}
+
+int swe_get_static_layout(SweContext* c, int32_t* layout) {
+  if (!c || !layout) return fail(SWE_EINVAL, "swe_get_static_layout: bad arguments");
+  double v = 0.0;
+  SWE_HIP_TRY(hipSetDevice(c->device));
+  SWE_HIP_TRY(hipMemcpy(&v, c->d_sum_mult + 3, sizeof(double), hipMemcpyDeviceToHost));
+  *layout = (int32_t)v;
   return SWE_OK;
 }
 

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_double, c_float, c_int32, c_int64, c_void_p
+from ctypes import POINTER, byref, c_char_p, c_double, c_float, c_int32, c_int64, c_void_p
 
 import torch  # noqa: F401  (load torch's HIP runtime before the library)
 
@@ -99,6 +99,8 @@ SIGNATURES = {
     "swe_set_graph_chunk": (c_int32, [c_void_p, c_int32]),
     "swe_set_dt_log": (c_int32, [c_void_p, c_void_p, c_int32]),
     "swe_set_loop_finish": (c_int32, [c_void_p, c_int32]),
+    "swe_set_static_classes": (c_int32, [c_void_p, c_int32]),
+    "swe_get_static_layout": (c_int32, [c_void_p, POINTER(c_int32)]),
     "swe_set_sweep_order": (c_int32, [c_void_p, c_int32]),
     "swe_adjoint_step": (c_int32, [c_void_p, SweState, SweState, SweState, c_double, c_double,
                                    c_void_p, c_void_p, c_void_p]),
@@ -299,6 +301,16 @@ class Context:
 
     def set_loop_finish(self, split: int) -> None:
         check(load().swe_set_loop_finish(self.handle, int(split)), "swe_set_loop_finish")
+
+    def set_static_classes(self, enable: int) -> None:
+        check(load().swe_set_static_classes(self.handle, int(enable)), "swe_set_static_classes")
+
+    def static_layout(self) -> int:
+        """Static record chosen at the last fast device-loop start: 2 = bed +
+        Manning class, 1 = (bed, n^2), 0 = (bed, n^2, multiplier)."""
+        v = c_int32(-1)
+        check(load().swe_get_static_layout(self.handle, byref(v)), "swe_get_static_layout")
+        return int(v.value)
 
     def set_sweep_order(self, alternate: int) -> None:
         check(load().swe_set_sweep_order(self.handle, int(alternate)), "swe_set_sweep_order")

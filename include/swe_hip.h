@@ -135,6 +135,24 @@ int swe_set_adjoint_rows(SweContext* ctx, int32_t rows);
  * (one launch per step, but a release/acquire across workgroups).  All three
  * give the same states, step sizes and FP64 balance. */
 int swe_set_loop_finish(SweContext* ctx, int32_t split);
+/* Static record of the fast device loop (tuning knob).  With enable = 1
+ * (default) a loop start whose rain multiplier is uniform over the active
+ * cells and whose active cells carry at most 255 distinct Manning values
+ * stores each cell as its bed elevation and a one-byte class (5 bytes), the
+ * step reading n^2 from a 255-entry table; otherwise (bed, n^2) with the
+ * multiplier as a constant (8 bytes) when it is uniform, else (bed, n^2,
+ * multiplier) (12 bytes).  All give bit-identical states.  Refused while a
+ * device loop is active.  No reference counterpart: the reference keeps the
+ * Manning raster as a dense float array (src/hybrid_flood/jax_solver/
+ * numerics.py:33, applied at :359), rasterized from the roughness of a
+ * land-use class layer (src/hybrid_flood/structured_io.py:103-113, lookup
+ * src/hybrid_flood/anuga_solver/build_domain.py:187-205), so it holds a few
+ * distinct values. */
+int swe_set_static_classes(SweContext* ctx, int32_t enable);
+/* The static record the last fast device loop start chose: 2 = bed + class,
+ * 1 = (bed, n^2), 0 = (bed, n^2, multiplier).  Blocks until the device has
+ * written it.  No reference counterpart. */
+int swe_get_static_layout(SweContext* ctx, int32_t* layout);
 /* Strip order of the device loop (tuning knob): 1 (default) = odd steps
  * sweep the strips in reverse, so each step starts where the previous one
  * ended and reads what is still in the memory-side cache; 0 = every step

@@ -8,7 +8,8 @@ schedule with ``finite_volume_step`` must give the same bits -- also with
 negative Manning coefficients (the packed copy stores n^2, as the friction law
 uses it) -- and fields changed in place between two runs must be used by the
 second run.  The loop's alternating strip order (odd steps sweep in reverse)
-must not change a bit either."""
+must not change a bit either.  Nor may the static record's form: a one-byte
+Manning class, (bed, n^2), or (bed, n^2, multiplier)."""
 
 from __future__ import annotations
 
@@ -228,3 +229,141 @@ def test_degenerate_shapes_tile_invariant_and_near_oracle(cuda, shape):
     for f, atol in (("h", 1e-5), ("hu", 5e-5), ("hv", 5e-5)):
         got = getattr(base.states, f)[-1].cpu().numpy().astype(np.float64)
         assert np.abs(got - getattr(st, f)).max() <= atol, f
+
+
+
+def _palette_manning(mask, n_values, rng):
+    """Manning values drawn from ``n_values`` distinct float32 bit patterns on
+    the active cells (0.0 and -0.0 among them, negative values included, each
+    on some cell), and 7.5, a value outside the palette, on the inactive ones
+    (never read there: counted, it would make 256 classes of 255)."""
+    pal = rng.uniform(0.01, 0.2, max(n_values, 2)).astype(np.float32)
+    pal[rng.uniform(size=pal.size) < 0.3] *= -1.0
+    pal[0], pal[1] = np.float32(0.0), np.float32(-0.0)
+    pal = pal[:n_values]
+    assert np.unique(pal.view(np.uint32)).size == n_values
+    idx = rng.integers(0, n_values, mask.shape)
+    active = np.flatnonzero(mask)
+    assert active.size >= n_values
+    idx.ravel()[active[:n_values]] = np.arange(n_values)
+    return np.where(mask, pal[idx], np.float32(7.5)).astype(np.float32)
+
+
+@pytest.mark.parametrize("n_values,mult,layout", [(1, "uniform", 2), (37, "mask", 2),
+                                                  (255, "uniform", 2), (256, "uniform", 1),
+                                                  (37, "field", 0)])
+def test_static_record_forms_give_the_same_bits(cuda, n_values, mult, layout):
+    """The loop start stores the bed and a one-byte Manning class when the
+    multiplier is uniform and the active cells carry at most 255 distinct
+    Manning bit patterns, else (bed, n^2) or (bed, n^2, multiplier); every
+    form gives the bits of the 8- or 12-byte record and of a replay of the
+    loop's dt schedule with the caller-layout step."""
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd.adjoint import rollout_forward
+
+    rng = np.random.default_rng(100 + n_values)
+    case = random_case(41, (70, 133), open_sides=(0, 1, 0, 1), speed=0.8, rain=2.0e-4)
+    case["manning_n"] = _palette_manning(case["domain_mask"], n_values, rng)
+    case = _set_multiplier(case, mult, rng)
+    params = _params(case, cuda)
+    ctx = nm.context_for(params, "fast")
+    times = np.array([0.0, 1.5, 4.0], np.float32)
+    runs, layouts = [], []
+    try:
+        for enable in (1, 0):
+            ctx.set_static_classes(enable)
+            runs.append(rollout_forward(params, _state(case, cuda), times, max_steps=3000,
+                                        variant="fast", record=False))
+            layouts.append(ctx.static_layout())
+    finally:
+        ctx.set_static_classes(1)
+    print(f"classes {n_values}, multiplier {mult}: layouts {layouts}", flush=True)
+    assert layouts == [layout, min(layout, 1)]
+    (_, with_cls, tape), (_, without, tape2) = runs
+    assert tape.steps == tape2.steps > 10
+    for a, b in zip(with_cls.states, without.states):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    cur = _state(case, cuda)
+    for k in range(tape.steps):
+        cur = nm.finite_volume_step(cur, params, tape.dts[k], tape.ts[k], variant="fast")
+    for got, want in zip((with_cls.states.h[-1], with_cls.states.hu[-1], with_cls.states.hv[-1]),
+                         cur):
+        assert np.array_equal(got.cpu().numpy(), want.cpu().numpy())
+    with pytest.raises(ValueError):
+        ctx.set_static_classes(2)
+
+
+def test_manning_classes_follow_in_place_changes(cuda):
+    """A Manning field changed in place between runs re-chooses the record:
+    300 values fall back to (bed, n^2), a new 12-value palette returns to
+    classes, and every run equals a fresh context's."""
+    import torch
+
+    from hybrid_flood_amd import numerics as nm
+
+    rng = np.random.default_rng(7)
+    case = _set_multiplier(random_case(43, (50, 90), open_sides=(1, 0, 0, 0), speed=0.8),
+                           "uniform", rng)
+    mask = case["domain_mask"]
+    case["manning_n"] = _palette_manning(mask, 20, rng)
+    params = _params(case, cuda)
+    ctx = nm.context_for(params, "fast")
+    times = np.array([0.0, 3.0], np.float32)
+    for n_values, layout in ((20, 2), (300, 1), (12, 2)):
+        if n_values != 20:
+            case["manning_n"] = _palette_manning(mask, n_values, rng)
+            params.manning_n.copy_(torch.as_tensor(case["manning_n"], device=cuda))
+        got = nm.integrate_to_outputs(_state(case, cuda), params, times, max_steps=2000,
+                                      variant="fast")
+        assert ctx.static_layout() == layout
+        fresh = nm.integrate_to_outputs(_state(case, cuda), _params(case, cuda), times,
+                                        max_steps=2000, variant="fast")
+        for a, b in zip(got.states, fresh.states):
+            assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
+@pytest.mark.parametrize("variant", ["fast", "reference"])
+def test_loop_finish_modes_agree(cuda, variant):
+    """The three ways a device-loop step closes (fused: the next step derives
+    dt from CFL slots; split: a finish kernel after each step; last
+    workgroup: the step kernel's last workgroup finishes) give the same
+    states, dt schedule and FP64 diagnostics; the mode cannot change while a
+    loop is active."""
+    from hybrid_flood_amd import _lib
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd.adjoint import rollout_forward
+
+    case = random_case(53, (75, 140), open_sides=(1, 1, 0, 0), speed=0.8, rain=2.0e-4)
+    params = _params(case, cuda)
+    ctx = nm.context_for(params, variant)
+    times = np.array([0.0, 1.0, 3.0], np.float32)
+    runs = {}
+    try:
+        for mode in (1, 2, 0):
+            ctx.set_loop_finish(mode)
+            print(f"finish mode {mode}, variant {variant}", flush=True)
+            runs[mode] = rollout_forward(params, _state(case, cuda), times, max_steps=3000,
+                                         variant=variant, record=False)
+    finally:
+        ctx.set_loop_finish(1)
+    _, base, tape = runs[1]
+    assert tape.steps > 10
+    for mode in (2, 0):
+        _, r, t = runs[mode]
+        assert t.steps == tape.steps
+        assert np.array_equal(np.asarray(t.dts), np.asarray(tape.dts))
+        for a, b in zip(r.states, base.states):
+            assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+        for key in ("mass_balance_rel", "rain_volume", "boundary_volume", "clip_volume",
+                    "volume_final"):
+            assert r.diagnostics[key] == base.diagnostics[key], key
+    with pytest.raises(ValueError):
+        ctx.set_loop_finish(3)
+    st = _state(case, cuda)
+    lib = _lib.load()
+    _lib.check(lib.swe_loop_begin(ctx.handle, _lib.state_struct(*st), 0.0, 4, _lib.stream_handle()),
+               "swe_loop_begin")
+    with pytest.raises(ValueError):
+        ctx.set_loop_finish(2)
+    _lib.check(lib.swe_loop_end(ctx.handle, _lib.state_struct(*st), None, _lib.stream_handle()),
+               "swe_loop_end")
