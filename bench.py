@@ -53,6 +53,11 @@ BYTES_PER_CELL = 37  # h,hu,hv,bed,n,mult read (24) + mask (1) + h,hu,hv written
 HBM_PEAK_GBS = 8000.0
 
 
+def _progress(msg: str) -> None:
+    """One line per finished phase on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def _even_divisor(k: int, cap: int = 64) -> int:
     for c in range(min(cap, k), 1, -1):
         if k % c == 0 and c % 2 == 0:
@@ -126,8 +131,9 @@ def time_loop(params, h0, steps, warmup, settle, segments, variant, rows=None):
 
 def stream_probes(cells: int, reps: int = 10) -> dict:
     """Achievable bandwidth of this device, measured with the library's own
-    streaming kernels: a copy, a read, and the packed step's three streams
-    (12 B + 8 B read, 12 B written per cell) over the step's cell count."""
+    streaming kernels: a copy, a read, and the packed step's streams (12 B of
+    state and 8 B, or 5 B with Manning classes, of static record read, 12 B
+    written per cell) over the step's cell count."""
     import ctypes
 
     import torch
@@ -164,8 +170,15 @@ def stream_probes(cells: int, reps: int = 10) -> dict:
     dst = torch.empty(3 * cells, device="cuda")
     out["step_streams"] = run(2, vp(st.data_ptr()), vp(sx.data_ptr()), vp(dst.data_ptr()), cells,
                               32 * cells)
-    out["step_streams"]["note"] = ("the packed step's streams (32 B per cell) without its "
-                                   "arithmetic: the memory-only time of one step")
+    out["step_streams"]["note"] = ("the packed step's streams with the (bed, n^2) record "
+                                   "(32 B per cell) without its arithmetic")
+    del sx
+    sc = torch.ones(((cells + 63) // 64) * 80, device="cuda")  # 320-byte blocks
+    out["step_streams_class"] = run(3, vp(st.data_ptr()), vp(sc.data_ptr()), vp(dst.data_ptr()),
+                                    cells, 29 * cells)
+    out["step_streams_class"]["note"] = ("the packed step's streams with the class-indexed record "
+                                         "(29 B per cell) without its arithmetic: the memory-only "
+                                         "time of one step of this workload")
     return out
 
 
@@ -190,7 +203,8 @@ def config1_gpu_run(variant: str) -> dict:
     _, res, tape = rollout_forward(params, nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h)),
                                    times, max_steps=20000, variant=variant, record=False)
     return {"case": c, "times": times, "states": [x.cpu().numpy() for x in res.states],
-            "dts": tape.dts, "steps": int(tape.steps), "diagnostics": res.diagnostics}
+            "dts": tape.dts, "steps": int(tape.steps), "diagnostics": res.diagnostics,
+            "variant": variant}
 
 
 def cpu_baseline(case, steps: int, cfg1: dict, reps: int) -> tuple[dict, dict]:
@@ -202,7 +216,6 @@ def cpu_baseline(case, steps: int, cfg1: dict, reps: int) -> tuple[dict, dict]:
     import numpy as np
 
     sys.path.insert(0, ROOT)
-    from oracle import accuracy as acc
     from oracle import swe_oracle as o
 
     p = o.OracleParams(bed=case.bed, manning_n=case.manning_n, domain_mask=case.domain_mask,
@@ -213,6 +226,7 @@ def cpu_baseline(case, steps: int, cfg1: dict, reps: int) -> tuple[dict, dict]:
     t0 = time.perf_counter()
     o.run_steps(o.OracleState(case.h0, z, z), p, steps)
     el = time.perf_counter() - t0
+    _progress(f"cpu baseline: {steps} oracle steps at {case.bed.shape[0]}^2, {el:.1f} s")
     n = case.bed.size
     c1 = cfg1["case"]
     op1 = o.OracleParams(bed=c1.bed, manning_n=c1.manning_n, domain_mask=c1.domain_mask,
@@ -227,6 +241,7 @@ def cpu_baseline(case, steps: int, cfg1: dict, reps: int) -> tuple[dict, dict]:
         r1 = o.integrate_to_outputs(o.OracleState(c1.h0, z1, z1), op1, cfg1["times"],
                                     max_steps=20000)
         secs.append(time.perf_counter() - t1)
+        _progress(f"cpu baseline: configuration 1 rep {len(secs)}/{reps}, {secs[-1]:.1f} s")
     med = float(np.median(secs))
     base = {
         "value": n * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
@@ -240,12 +255,29 @@ def cpu_baseline(case, steps: int, cfg1: dict, reps: int) -> tuple[dict, dict]:
             "cell_updates_per_s": c1.bed.size * r1.steps_used / med,
         },
     }
-    rec = acc.record(cfg1["states"], op1, c1.h0, cfg1["times"], cfg1["dts"])
-    rec["workload"] = "256x256 sloped plane (config 1), 600 s, 11 outputs, GPU fast variant"
-    rec["gpu_steps"] = cfg1["steps"]
+    rec = accuracy_record(cfg1)
     rec["oracle_own_schedule_steps"] = int(r1.steps_used)
-    rec["gpu_mass_balance_rel"] = cfg1["diagnostics"]["mass_balance_rel"]
     return base, rec
+
+
+def accuracy_record(cfg1: dict) -> dict:
+    """Accuracy record of a GPU run of configuration 1 against the oracle
+    replayed along that run's dt schedule in float32 and float64."""
+    sys.path.insert(0, ROOT)
+    from oracle import accuracy as acc
+    from oracle import swe_oracle as o
+
+    c1 = cfg1["case"]
+    op1 = o.OracleParams(bed=c1.bed, manning_n=c1.manning_n, domain_mask=c1.domain_mask,
+                         boundary_types=c1.boundary_types, rainfall_times_s=c1.rain_times_s,
+                         rainfall_rates_m_s=c1.rain_rates_m_s, dx=c1.dx, cfl=c1.cfl,
+                         dry_tolerance=c1.dry_tolerance, max_dt=c1.max_dt)
+    rec = acc.record(cfg1["states"], op1, c1.h0, cfg1["times"], cfg1["dts"])
+    rec["workload"] = ("256x256 sloped plane (config 1), 600 s, 11 outputs, GPU "
+                       f"{cfg1['variant']} variant")
+    rec["gpu_steps"] = cfg1["steps"]
+    rec["gpu_mass_balance_rel"] = cfg1["diagnostics"]["mass_balance_rel"]
+    return rec
 
 
 def config2_mass_balance(variant: str) -> dict:
@@ -594,8 +626,10 @@ def main(argv=None) -> int:
                             rainfall_rates_m_s=case.rain_rates_m_s, dx=case.dx, cfl=case.cfl,
                             dry_tolerance=case.dry_tolerance, max_dt=case.max_dt, device=dev)
     cells = case.bed.size
+    _progress(f"case {args.size}^2 built; tuning and timing the loop")
     r = time_loop(params, case.h0, args.steps, args.warmup, args.settle, args.segments,
                   args.variant, args.rows)
+    _progress(f"timed loop: {r['ms'] / args.steps * 1e3:.1f} us per step, tile {r['tile']}")
     ms_per_step = r["ms"] / args.steps
     value = cells * args.steps / (r["ms"] * 1e-3)
     achieved = BYTES_PER_CELL * cells / (ms_per_step * 1e-3) / 1e9
@@ -642,8 +676,12 @@ def main(argv=None) -> int:
             t = json.load(f)
         line["roofline"]["traffic"] = t.get("hbm_bytes_per_launch")
         line["roofline"]["traffic_source"] = "profiles/step_fast_4096_traffic.json (rocprofv3 --pmc)"
+        if "valu" in t:  # the step is co-limited by vector-instruction issue
+            line["roofline"]["valu_busy_frac"] = t["valu"]["valu_busy_frac"]
+            line["roofline"]["valu_per_wavefront_row"] = t["valu"]["valu_per_wavefront_row"]
     if args.probes > 0:
         line["bandwidth_probes"] = stream_probes(cells)
+        _progress("bandwidth probes done")
     if args.extra:
         other = "reference" if args.variant == "fast" else "fast"
         k = min(args.steps, 400)
@@ -660,20 +698,28 @@ def main(argv=None) -> int:
         c2 = config2_mass_balance(args.variant)
         mb["config2"] = c2["mass_balance_rel"]
         line["config2_run"] = c2
+        _progress("configurations 1 and 2 done")
     if args.ensemble > 0:
         line["ensemble"] = bench_ensemble(args.variant, args.ensemble, 42, args.ensemble_t, value)
         mb["config4_max_abs_over_members"] = line["ensemble"]["max_abs_mass_balance_rel"]
+        _progress("ensemble done")
     if args.adjoint_n > 0:
         line["adjoint"] = bench_adjoint(args.variant, args.adjoint_n, args.adjoint_steps, value)
         mb["config5_forward"] = line["adjoint"]["forward_mass_balance_rel"]
+        _progress("adjoint done")
     line["mass_balance_rel_by_config"] = mb
     if args.callers > 0:
         line["callers"] = bench_callers(params, case)
+        _progress("callers done")
     if args.cpu_steps > 0:
         if cfg1 is None:
             cfg1 = config1_gpu_run(args.variant)
+        _progress("cpu baseline: timing the oracle on the host")
         line["cpu_baseline"], line["accuracy"] = cpu_baseline(case, args.cpu_steps, cfg1,
                                                               args.cpu_reps)
+        other = "reference" if args.variant == "fast" else "fast"
+        line[f"accuracy_{other}_variant"] = accuracy_record(config1_gpu_run(other))
+        _progress(f"accuracy record of the {other} variant done")
     print(json.dumps(line), flush=True)
     return 0
 

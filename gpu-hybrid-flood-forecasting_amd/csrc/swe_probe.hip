@@ -8,6 +8,9 @@
 //           cell, 12 bytes of packed state and 8 bytes of packed static
 //           fields read and 12 bytes of packed state written, one cell per
 //           lane, as the step's 96- and 64-bit accesses
+//   mode 3  as mode 2 with the class-indexed static record: 4 bytes of bed
+//           and 1 byte of class per cell, in blocks of 64 cells (64 beds,
+//           then 64 classes: 320 bytes), src2 holding ceil(n / 64) blocks
 // One element per thread; the caller times the launches with events.
 
 #include <hip/hip_runtime.h>
@@ -59,6 +62,19 @@ __global__ __launch_bounds__(kProbeBlock) void k_probe_stepmix(const Cell3* __re
   }
 }
 
+__global__ __launch_bounds__(kProbeBlock) void k_probe_stepcls(const Cell3* __restrict__ st,
+                                                               const unsigned char* __restrict__ sx,
+                                                               Cell3* __restrict__ out, size_t n) {
+  const size_t k = (size_t)blockIdx.x * kProbeBlock + threadIdx.x;
+  if (k < n) {
+    const Cell3 u = st[k];
+    const unsigned char* blk = sx + k / 64 * 320;
+    const float b = reinterpret_cast<const float*>(blk)[k % 64];
+    const unsigned c = blk[256 + k % 64];
+    out[k] = Cell3{u.a + b, u.b + (float)c, u.c};
+  }
+}
+
 }  // namespace
 }  // namespace swe
 
@@ -66,10 +82,10 @@ using namespace swe;
 
 extern "C" int swe_stream_probe(int32_t mode, const void* src, const void* src2, void* dst,
                                 int64_t n, void* stream) {
-  if (n <= 0 || !src || !dst || mode < 0 || mode > 2 || (mode == 2 && !src2))
+  if (n <= 0 || !src || !dst || mode < 0 || mode > 3 || (mode >= 2 && !src2))
     return set_error(SWE_EINVAL, "swe_stream_probe: bad arguments");
   if ((mode < 2 && (n % 4 != 0 || ((uintptr_t)src & 15) || ((uintptr_t)dst & 15))) ||
-      (mode == 2 && (((uintptr_t)src & 3) || ((uintptr_t)src2 & 7) || ((uintptr_t)dst & 3))))
+      (mode >= 2 && (((uintptr_t)src & 3) || ((uintptr_t)src2 & 7) || ((uintptr_t)dst & 3))))
     return set_error(SWE_EINVAL, "swe_stream_probe: misaligned buffers or n not a multiple of 4");
   hipStream_t s = (hipStream_t)stream;
   const size_t items = mode < 2 ? (size_t)n / 4 : (size_t)n;
@@ -80,9 +96,12 @@ extern "C" int swe_stream_probe(int32_t mode, const void* src, const void* src2,
     hipLaunchKernelGGL(k_probe_copy, grid, block, 0, s, (const float4*)src, (float4*)dst, items);
   else if (mode == 1)
     hipLaunchKernelGGL(k_probe_read, grid, block, 0, s, (const float4*)src, (float*)dst, items);
-  else
+  else if (mode == 2)
     hipLaunchKernelGGL(k_probe_stepmix, grid, block, 0, s, (const Cell3*)src, (const Cell2*)src2,
                        (Cell3*)dst, items);
+  else
+    hipLaunchKernelGGL(k_probe_stepcls, grid, block, 0, s, (const Cell3*)src,
+                       (const unsigned char*)src2, (Cell3*)dst, items);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(SWE_EHIP, std::string("swe_stream_probe: ") + hipGetErrorString(e));
   return SWE_OK;

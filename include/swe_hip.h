@@ -426,7 +426,9 @@ int swe_correct_depth(int32_t count, int32_t ny, int32_t nx, const float* depth,
  *           float per 1024 read to dst (ceil(n / 1024) floats);
  *   mode 2: the packed device loop's streams without its arithmetic: n cells,
  *           src = 3n floats of packed state, src2 = 2n floats of packed static
- *           fields, dst = 3n floats written. */
+ *           fields, dst = 3n floats written;
+ *   mode 3: as mode 2 with the class-indexed static record: src2 = ceil(n / 64)
+ *           blocks of 320 bytes (64 float beds, then 64 one-byte classes). */
 int swe_stream_probe(int32_t mode, const void* src, const void* src2, void* dst, int64_t n,
                      void* stream);
 
