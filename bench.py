@@ -129,6 +129,68 @@ def time_loop(params, h0, steps, warmup, settle, segments, variant, rows=None):
                      if tuned else {"rows": rows, "waves_per_cu": 0, "autotuned": False})}
 
 
+def window_probe(params, h0, variant: str, rows, idle_s: float = 0.5, seg: int = 2,
+                 segments: int = 200, per_window: int = 10) -> dict:
+    """Step time and average shader clock in windows of ``seg * per_window``
+    steps: after ``idle_s`` seconds of an idle GPU, and then again under
+    continuous load.  Between segments of the device loop a one-thread probe
+    kernel samples the shader clock counter and the real-time counter on the
+    same stream, so each window's clock is (cycles elapsed) / (time elapsed)
+    on the device itself, without a profiler."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from hybrid_flood_amd import _lib
+    from hybrid_flood_amd import numerics as nm
+
+    dev = params.bed.device
+    lib = _lib.load()
+    ctx = nm.context_for(params, variant)
+    if rows:
+        ctx.set_tile_rows(rows)
+    ctx.set_graph_chunk(_even_divisor(seg))
+    h = torch.as_tensor(h0, device=dev).contiguous()
+    state = nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h))
+    s = _lib.stream_handle(dev)
+    st = _lib.state_struct(*state)
+    total = 400 + 2 * seg * segments
+    _lib.check(lib.swe_loop_begin(ctx.handle, st, 0.0, total, s), "swe_loop_begin")
+    _lib.check(lib.swe_loop_advance(ctx.handle, 400, s), "swe_loop_advance")
+    clk = torch.zeros(2 * (segments + 1), dtype=torch.int64, device=dev)
+    vp = ctypes.c_void_p
+    out = {}
+    for phase, idle in (("after_idle", idle_s), ("continuous", 0.0)):
+        torch.cuda.synchronize(dev)
+        if idle:
+            time.sleep(idle)
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(segments + 1)]
+        _lib.check(lib.swe_stream_probe(4, None, None, vp(clk.data_ptr()), 1, s), "clock probe")
+        evs[0].record()
+        for k in range(segments):
+            _lib.check(lib.swe_loop_advance(ctx.handle, seg, s), "swe_loop_advance")
+            evs[k + 1].record()
+            _lib.check(lib.swe_stream_probe(4, None, None, vp(clk.data_ptr()), k + 2, s),
+                       "clock probe")
+        torch.cuda.synchronize(dev)
+        c = clk.cpu().numpy().reshape(-1, 2).astype(np.float64)
+        ms = np.array([evs[k].elapsed_time(evs[k + 1]) for k in range(segments)])
+        wall_rate = (c[-1, 1] - c[0, 1]) / (ms.sum() * 1e-3)  # real-time counter ticks per s
+        wins = []
+        for w in range(segments // per_window):
+            a, b = w * per_window, (w + 1) * per_window
+            dt_s = (c[b, 1] - c[a, 1]) / wall_rate
+            wins.append({"steps": seg * per_window,
+                         "us_per_step": float(ms[a:b].sum() / (seg * per_window) * 1e3),
+                         "shader_mhz": float((c[b, 0] - c[a, 0]) / dt_s / 1e6)})
+        out[phase] = {"idle_s": idle, "realtime_counter_hz": float(wall_rate), "windows": wins}
+        _progress(f"window probe: {phase} done")
+    info = _lib.SweRunInfo()
+    _lib.check(lib.swe_loop_end(ctx.handle, st, ctypes.byref(info), s), "swe_loop_end")
+    return out
+
+
 def stream_probes(cells: int, reps: int = 10) -> dict:
     """Achievable bandwidth of this device, measured with the library's own
     streaming kernels: a copy, a read, and the packed step's streams (12 B of
@@ -597,6 +659,9 @@ def main(argv=None) -> int:
     ap.add_argument("--probes", type=int, default=1, help="streaming bandwidth probes (0 = skip)")
     ap.add_argument("--configs", type=int, default=1,
                     help="run configurations 1 and 2 for their mass balance (0 = skip)")
+    ap.add_argument("--window-probe", action="store_true",
+                    help="only print the step time and shader clock per 20-step window after an "
+                         "idle gap and under continuous load (the 4096^2 loop) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 device copies of 256 MiB (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -626,6 +691,10 @@ def main(argv=None) -> int:
                             rainfall_rates_m_s=case.rain_rates_m_s, dx=case.dx, cfl=case.cfl,
                             dry_tolerance=case.dry_tolerance, max_dt=case.max_dt, device=dev)
     cells = case.bed.size
+    if args.window_probe:
+        print(json.dumps({"window_probe": window_probe(params, case.h0, args.variant,
+                                                       args.rows or 21)}), flush=True)
+        return 0
     _progress(f"case {args.size}^2 built; tuning and timing the loop")
     r = time_loop(params, case.h0, args.steps, args.warmup, args.settle, args.segments,
                   args.variant, args.rows)

@@ -11,6 +11,10 @@
 //   mode 3  as mode 2 with the class-indexed static record: 4 bytes of bed
 //           and 1 byte of class per cell, in blocks of 64 cells (64 beds,
 //           then 64 classes: 320 bytes), src2 holding ceil(n / 64) blocks
+//   mode 4  clock sample: one thread writes the shader clock counter and the
+//           constant-rate real-time counter (two uint64) to dst slot n - 1;
+//           two samples on the stream around other work give the average
+//           shader clock over it
 // One element per thread; the caller times the launches with events.
 
 #include <hip/hip_runtime.h>
@@ -75,6 +79,13 @@ __global__ __launch_bounds__(kProbeBlock) void k_probe_stepcls(const Cell3* __re
   }
 }
 
+__global__ __launch_bounds__(64) void k_probe_clock(unsigned long long* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    out[0] = (unsigned long long)clock64();
+    out[1] = (unsigned long long)wall_clock64();
+  }
+}
+
 }  // namespace
 }  // namespace swe
 
@@ -82,8 +93,17 @@ using namespace swe;
 
 extern "C" int swe_stream_probe(int32_t mode, const void* src, const void* src2, void* dst,
                                 int64_t n, void* stream) {
-  if (n <= 0 || !src || !dst || mode < 0 || mode > 3 || (mode >= 2 && !src2))
+  if (n <= 0 || !dst || mode < 0 || mode > 4 || (mode < 4 && !src) ||
+      ((mode == 2 || mode == 3) && !src2))
     return set_error(SWE_EINVAL, "swe_stream_probe: bad arguments");
+  if (mode == 4) {
+    if ((uintptr_t)dst & 7) return set_error(SWE_EINVAL, "swe_stream_probe: misaligned buffers");
+    hipLaunchKernelGGL(k_probe_clock, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                       static_cast<unsigned long long*>(dst) + 2 * (size_t)(n - 1));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(SWE_EHIP, std::string("swe_stream_probe: ") + hipGetErrorString(e));
+    return SWE_OK;
+  }
   if ((mode < 2 && (n % 4 != 0 || ((uintptr_t)src & 15) || ((uintptr_t)dst & 15))) ||
       (mode >= 2 && (((uintptr_t)src & 3) || ((uintptr_t)src2 & 7) || ((uintptr_t)dst & 3))))
     return set_error(SWE_EINVAL, "swe_stream_probe: misaligned buffers or n not a multiple of 4");

@@ -428,7 +428,12 @@ int swe_correct_depth(int32_t count, int32_t ny, int32_t nx, const float* depth,
  *           src = 3n floats of packed state, src2 = 2n floats of packed static
  *           fields, dst = 3n floats written;
  *   mode 3: as mode 2 with the class-indexed static record: src2 = ceil(n / 64)
- *           blocks of 320 bytes (64 float beds, then 64 one-byte classes). */
+ *           blocks of 320 bytes (64 float beds, then 64 one-byte classes);
+ *   mode 4: clock sample (src, src2 unused): writes the shader clock counter
+ *           and the constant-rate real-time counter (hipDeviceAttributeWallClockRate)
+ *           as two uint64 to dst slot n - 1 (dst 8-byte aligned, n >= 1); two
+ *           samples around other work on the stream give its average shader
+ *           clock. */
 int swe_stream_probe(int32_t mode, const void* src, const void* src2, void* dst, int64_t n,
                      void* stream);
 

@@ -50,7 +50,9 @@ def test_built_library_has_no_packed_fp32_instructions():
 def test_stream_probe_argument_errors():
     L = _lib()
     lib = L.load()
-    assert lib.swe_stream_probe(4, 16, 16, 16, 8, None) == L.SWE_EINVAL
+    assert lib.swe_stream_probe(5, 16, 16, 16, 8, None) == L.SWE_EINVAL
+    assert lib.swe_stream_probe(4, None, None, 12, 1, None) == L.SWE_EINVAL  # dst misaligned
+    assert lib.swe_stream_probe(4, None, None, 16, 0, None) == L.SWE_EINVAL  # slot n - 1 < 0
     assert lib.swe_stream_probe(-1, 16, 16, 16, 8, None) == L.SWE_EINVAL
     assert lib.swe_stream_probe(0, 16, None, 16, 6, None) == L.SWE_EINVAL  # n % 4
     assert lib.swe_stream_probe(2, 16, None, 16, 8, None) == L.SWE_EINVAL  # src2 missing
