@@ -24,6 +24,9 @@ implements the raster operations in NumPy:
   when its centre is, edges half-open), and later features overwrite
   earlier ones, as rasterio's ``rasterize``/``geometry_mask`` do with
   ``all_touched=False``;
+* a roughness layer in another CRS than the domain's is moved into it vertex
+  by vertex (``projections``: WGS 84 geographic, Pseudo-Mercator and UTM, the
+  reference's ``to_crs``); other CRS pairs are refused;
 * resampling handles one CRS only (the reference also refuses a DEM whose CRS
   differs from the domain's); "bilinear" is a separable tent whose support
   widens with the downsampling factor, renormalised over valid source cells,
@@ -47,6 +50,7 @@ from typing import Any, Iterable, Sequence
 
 import numpy as np
 
+from . import projections
 from .shallow_water_2d import StructuredGrid
 
 MM_HR_TO_M_S = 1.0e-3 / 3600.0
@@ -676,8 +680,14 @@ def load_structured_grid(dem_path: str | Path, roughness_path: str | Path,
     if domain.crs is None or domain.projected is False:
         raise ValueError("Domain must use a projected CRS.")
     if roughness.crs is not None and roughness.crs != domain.crs:
-        raise ValueError(f"Roughness CRS {roughness.crs} differs from the domain CRS {domain.crs}; "
-                         "reprojection is not available here.")
+        # the reference moves the layer with to_crs (shallow_water_2d.py:86)
+        if not (projections.is_supported(roughness.crs) and projections.is_supported(domain.crs)):
+            raise ValueError(f"Roughness CRS {roughness.crs} differs from the domain CRS "
+                             f"{domain.crs}; reprojection between them is not available here "
+                             "(EPSG:4326, EPSG:3857 and the WGS 84 UTM zones are).")
+        roughness = VectorLayer(
+            projections.transform_polygons(roughness.polygons, roughness.crs, domain.crs),
+            roughness.attributes, domain.crs, domain.projected, roughness.kinds)
     if "roughness" not in roughness.attributes:
         raise ValueError("LULC roughness layer must contain the 'roughness' attribute.")
     polys = domain_polygons(domain)

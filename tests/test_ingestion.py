@@ -432,3 +432,41 @@ def test_reference_module_paths_reexport_the_loaders():
 
     assert callable(shallow_water_2d.load_structured_grid)
     assert structured_io.load_rainfall_series is shallow_water_2d.load_rainfall_series
+
+
+def test_roughness_layer_in_another_crs_is_reprojected(tmp_path):
+    """A roughness layer in EPSG:4326 over a domain in EPSG:32643 is moved
+    into the domain's CRS vertex by vertex (the reference's to_crs,
+    jax_solver/shallow_water_2d.py:86): two lon/lat strips split at one
+    meridian give each cell the value of the side its centre's longitude
+    falls on.  A layer in a CRS this build cannot move is refused."""
+    from hybrid_flood_amd import projections as pj
+
+    src_tr = ig.NorthUpTransform.from_origin(1000.0, 5200.0, 2.0, 2.0)
+    X, Y = _centres((150, 140), src_tr)
+    write_geotiff(tmp_path / "dem.tif", _plane(X, Y).astype(np.float32), west=1000.0,
+                  north=5200.0, res=2.0, nodata=-9999.0)
+    box = [(1020.0, 4960.0), (1220.0, 4960.0), (1220.0, 5080.0), (1020.0, 5080.0)]
+    write_gpkg(tmp_path / "domain.gpkg", [_wkb_polygon([box])])
+    lon, lat = pj.transform(np.array([b[0] for b in box]), np.array([b[1] for b in box]),
+                            "EPSG:32643", "EPSG:4326")
+    split, _ = pj.transform(1120.5, 5020.0, "EPSG:32643", "EPSG:4326")
+    lo0, lo1, la0, la1 = lon.min() - 0.01, lon.max() + 0.01, lat.min() - 0.01, lat.max() + 0.01
+    west = [(lo0, la0), (float(split), la0), (float(split), la1), (lo0, la1)]
+    east = [(float(split), la0), (lo1, la0), (lo1, la1), (float(split), la1)]
+    write_gpkg(tmp_path / "rough.gpkg", [_wkb_polygon([west]), _wkb_polygon([east])],
+               {"roughness": [0.02, 0.06]}, epsg=4326, projected=False)
+    g = ig.load_structured_grid(tmp_path / "dem.tif", tmp_path / "rough.gpkg",
+                                tmp_path / "domain.gpkg", resolution_m=10.0)
+    assert g.crs == "EPSG:32643" and g.domain_mask.all()
+    cx, cy = np.meshgrid(g.x, g.y)
+    clon, _ = pj.transform(cx, cy, "EPSG:32643", "EPSG:4326")
+    assert np.abs(clon - split).min() > 1e-6  # no centre on the split
+    want = np.where(clon < split, np.float32(0.02), np.float32(0.06))
+    np.testing.assert_array_equal(g.manning_n, want)
+    assert (g.manning_n == np.float32(0.02)).any() and (g.manning_n == np.float32(0.06)).any()
+    write_gpkg(tmp_path / "rough_bng.gpkg", [_wkb_polygon([box])], {"roughness": [0.03]},
+               epsg=27700)
+    with pytest.raises(ValueError, match="reprojection between them is not available"):
+        ig.load_structured_grid(tmp_path / "dem.tif", tmp_path / "rough_bng.gpkg",
+                                tmp_path / "domain.gpkg", resolution_m=10.0)
