@@ -8,8 +8,14 @@ reference's ANUGA-compatible NetCDF schema (jax_solver/shallow_water_2d.py:
 outside the domain mask, velocities ``hu/h`` and ``hv/h`` where ``h > dry``,
 speed ``hypot(u, v)``; coordinates ``time`` (float64 s), ``x``, ``y``; the
 same attributes.  xarray and netCDF4 are not available here, so the result
-is a small ``ForecastDataset`` whose ``to_npz`` writes the same variables,
-coordinates and attributes (attributes as JSON) to one ``.npz`` file.
+is a small ``ForecastDataset``.  ``save_netcdf`` / ``ForecastDataset.to_netcdf``
+write it as a NetCDF file (structured_io.py:214-230) in the classic
+64-bit-offset format through SciPy's writer, which any NetCDF reader
+(xarray, netCDF4, ncdump) opens with the same variables, dimensions,
+coordinates and attributes; the reference's netCDF4/HDF5 encoding (zlib level
+4, 256 x 256 chunks) needs libraries absent here, so the data are stored
+uncompressed.  ``to_npz`` writes the same content to one ``.npz`` file
+(attributes as JSON).
 """
 
 from __future__ import annotations
@@ -53,6 +59,61 @@ class ForecastDataset:
                 "dims": {name: list(d) for name, (d, _) in self.data_vars.items()}}
         np.savez_compressed(p, __meta__=np.array(json.dumps(meta, default=str)), **arrays)
         return p
+
+
+    def to_netcdf(self, path: str | Path) -> Path:
+        """Write the dataset as a classic (64-bit offset) NetCDF file: coordinate
+        variables ``time`` (float64), ``x``, ``y``; data variables float32 with
+        their ``units``; the global attributes (those that are None are left out)."""
+        from scipy.io import netcdf_file
+
+        p = Path(path)
+        p.parent.mkdir(parents=True, exist_ok=True)
+        f = netcdf_file(str(p), "w", version=2)
+        try:
+            for name in ("time", "y", "x"):
+                v = np.asarray(self.coords[name])
+                f.createDimension(name, int(v.shape[0]))
+                var = f.createVariable(name, "f8", (name,))
+                var[:] = v.astype(np.float64)
+                for k, a in self.var_attrs.get(name, {}).items():
+                    setattr(var, k, a)
+            for name, (dims, t) in self.data_vars.items():
+                var = f.createVariable(name, "f4", tuple(dims))
+                var[:] = t.detach().cpu().numpy().astype(np.float32)
+                for k, a in self.var_attrs.get(name, {}).items():
+                    setattr(var, k, a)
+            for k, a in self.attrs.items():
+                if a is not None:
+                    setattr(f, k, a)
+        finally:
+            f.close()
+        return p
+
+
+def save_netcdf(dataset: ForecastDataset, output_path: str | Path) -> Path:
+    """structured_io.py:214-230: write ``dataset`` to ``output_path`` (parents created)."""
+    return dataset.to_netcdf(output_path)
+
+
+def load_netcdf(path: str | Path) -> dict:
+    """Variables (arrays), ``__attrs__`` and ``__var_attrs__`` of a file written by ``save_netcdf``."""
+    from scipy.io import netcdf_file
+
+    f = netcdf_file(str(path), "r", mmap=False)
+    try:
+        # NetCDF stores big-endian values; return native-order arrays
+        out = {name: np.asarray(v[:], dtype=v[:].dtype.newbyteorder("="))
+               for name, v in f.variables.items()}
+        out["__attrs__"] = {k: (v.decode() if isinstance(v, bytes) else v)
+                            for k, v in f._attributes.items()}
+        out["__var_attrs__"] = {name: {k: (a.decode() if isinstance(a, bytes) else a)
+                                       for k, a in v._attributes.items()}
+                                for name, v in f.variables.items()}
+        out["__dims__"] = {name: tuple(v.dimensions) for name, v in f.variables.items()}
+    finally:
+        f.close()
+    return out
 
 
 def load_npz(path: str | Path) -> dict:

@@ -250,6 +250,13 @@ def test_result_to_dataset_matches_oracle(cuda, tmp_path):
     back = load_npz(ds.to_npz(tmp_path / "out.npz"))
     assert np.array_equal(back["depth"], want["depth"], equal_nan=True)
     assert back["__meta__"]["var_attrs"]["velocity"]["units"] == "m s-1"
+    from hybrid_flood_amd.structured_io import load_netcdf, save_netcdf
+
+    nc = load_netcdf(save_netcdf(ds, tmp_path / "sub" / "out.nc"))
+    for name in ("elevation", "depth", "x_velocity", "y_velocity"):
+        assert np.array_equal(nc[name], want[name], equal_nan=True), name
+    assert nc["__dims__"]["depth"] == ("time", "y", "x")
+    assert nc["__attrs__"]["steps_used"] == res.steps_used
 
 
 @pytest.mark.parametrize("shape,offset", [((24, 36), 0), ((24, 36), 1), ((25, 37), 0), ((64, 4), 0)])
