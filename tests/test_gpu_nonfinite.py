@@ -46,7 +46,7 @@ def _run_cabi(case, cuda, variant, times, max_steps=2000):
         ctx.handle, _lib.state_struct(*st), t64.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), T,
         max_steps, cube[0].data_ptr(), cube[1].data_ptr(), cube[2].data_ptr(),
         _lib.SweState(None, None, None), ctypes.byref(info), _lib.stream_handle(cuda))
-    return rc, info, cube
+    return rc, info, cube, ctx
 
 
 @pytest.mark.parametrize("variant", ["fast", "reference"])
@@ -56,7 +56,7 @@ def test_nan_in_initial_depth_stops_at_once(cuda, variant):
     case = random_case(61, (40, 70), open_sides=(0, 1, 0, 0), speed=0.5)
     case["h"][12, 33] = np.nan
     print(f"nan initial depth, variant {variant}", flush=True)
-    rc, info, _ = _run_cabi(case, cuda, variant, [0.0, 1.0, 2.0])
+    rc, info, _, _ = _run_cabi(case, cuda, variant, [0.0, 1.0, 2.0])
     assert rc == _lib.SWE_ENONFINITE
     assert info.nonfinite == 1
     assert info.steps_used == 0 and info.outputs_written == 1
@@ -77,21 +77,29 @@ def test_solver_run_raises_on_nan_state(cuda):
         solver.run(st, np.array([0.0, 1.0], np.float32), max_steps=500)
 
 
-@pytest.mark.parametrize("variant", ["fast", "reference"])
-def test_nan_manning_stops_after_the_step_that_made_it(cuda, variant):
+@pytest.mark.parametrize("variant,palette", [("fast", False), ("reference", False),
+                                             ("fast", True)])
+def test_nan_manning_stops_after_the_step_that_made_it(cuda, variant, palette):
+    """palette: two Manning values and a uniform multiplier, so the fast loop
+    stores Manning classes and the NaN is a class of its own."""
     from hybrid_flood_amd import _lib
 
     case = random_case(63, (48, 80), open_sides=(1, 0, 0, 0), speed=0.5)
+    if palette:
+        case["manning_n"] = np.where(case["manning_n"] > 0.03, 0.035, 0.015).astype(np.float32)
+        case["rainfall_multiplier"] = case["domain_mask"].astype(np.float32)
     wet = np.argwhere(case["domain_mask"] & (case["h"] > 0.05))
     j, i = wet[len(wet) // 2]
     case["manning_n"] = case["manning_n"].copy()
     case["manning_n"][j, i] = np.nan
-    print(f"nan manning at ({j}, {i}), variant {variant}", flush=True)
-    rc, info, cube = _run_cabi(case, cuda, variant, [0.0, 50.0])
+    print(f"nan manning at ({j}, {i}), variant {variant}, palette {palette}", flush=True)
+    rc, info, cube, ctx = _run_cabi(case, cuda, variant, [0.0, 50.0])
     assert rc == _lib.SWE_ENONFINITE
     assert info.nonfinite == 1
     assert info.steps_used == 1
     assert info.final_time_s > 0.0
+    if palette:
+        assert ctx.static_layout() == 2
 
 
 def test_ensemble_member_with_nan_stops_alone(cuda):
