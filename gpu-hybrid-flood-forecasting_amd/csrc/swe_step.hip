@@ -1290,6 +1290,8 @@ int swe_get_static_layout(SweContext* c, int32_t* layout) {
   if (!c || !layout) return fail(SWE_EINVAL, "swe_get_static_layout: bad arguments");
   double v = 0.0;
   SWE_HIP_TRY(hipSetDevice(c->device));
+  // the loop start may have been queued on any stream of the device
+  SWE_HIP_TRY(hipDeviceSynchronize());
   SWE_HIP_TRY(hipMemcpy(&v, c->d_sum_mult + 3, sizeof(double), hipMemcpyDeviceToHost));
   *layout = (int32_t)v;
   return SWE_OK;

@@ -150,8 +150,8 @@ int swe_set_loop_finish(SweContext* ctx, int32_t split);
  * distinct values. */
 int swe_set_static_classes(SweContext* ctx, int32_t enable);
 /* The static record the last fast device loop start chose: 2 = bed + class,
- * 1 = (bed, n^2), 0 = (bed, n^2, multiplier).  Blocks until the device has
- * written it.  No reference counterpart. */
+ * 1 = (bed, n^2), 0 = (bed, n^2, multiplier).  Synchronises the device (every
+ * stream) before it reads the record.  No reference counterpart. */
 int swe_get_static_layout(SweContext* ctx, int32_t* layout);
 /* Strip order of the device loop (tuning knob): 1 (default) = odd steps
  * sweep the strips in reverse, so each step starts where the previous one
