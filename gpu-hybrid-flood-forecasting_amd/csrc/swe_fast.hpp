@@ -446,7 +446,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   const int ic = min(max(i, 0), G.nx - 1);
   const bool neg_hu = (i < 0 && G.bt[0] == 0) || (i >= G.nx && G.bt[1] == 0);
   // Only the first and last strip of a row band touch the west/east edge faces.
-  const bool edge_x = LOOP && (sx == 0 || (sx + 1) * kStripCols >= G.nx);
+  const int edge_x = LOOP && (sx == 0 || (sx + 1) * kStripCols >= G.nx);
   const int j0 = sy * rows;
   const int j1 = min(j0 + rows, G.ny);
   const float dry = G.dry;
@@ -556,7 +556,13 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     const float xl0 = from_east(fx.f0);
     const float xl1 = from_east(fx.fn + fx.cA);
     const float xl2 = from_east(fx.ft);
-    if (edge_x) {
+    // edge_x and the record range are uniform integers tested in every row
+    // through an empty asm statement (scalar register, no instruction): a
+    // loop-invariant test would be hoisted out of the march as a 64-bit lane
+    // mask, and with scalar registers short, spilled and re-expanded per row.
+    int ex = __builtin_amdgcn_readfirstlane(edge_x);
+    asm volatile("" : "+s"(ex));
+    if (ex) {
       if (i == 0) acc.bnd += (double)fx.f0 * dyd;
       if (i == G.nx) acc.bnd -= (double)fx.f0 * dyd;
     }
@@ -620,7 +626,10 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
       if (LOOP) {
         // Output or checkpoint slots (rare steps; rec_lo/rec_hi are
         // wave-uniform, so this is a scalar branch).
-        if (__builtin_expect(rec_lo < rec_hi, 0)) {
+        int lo = __builtin_amdgcn_readfirstlane(rec_lo);
+        int hi = __builtin_amdgcn_readfirstlane(rec_hi);
+        asm volatile("" : "+s"(lo), "+s"(hi));
+        if (__builtin_expect(lo < hi, 0)) {
           // j through an empty asm statement: keeps the compiler from
           // strength-reducing these addresses into per-lane pointers that
           // every row would advance.

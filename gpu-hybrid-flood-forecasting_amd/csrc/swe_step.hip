@@ -76,8 +76,15 @@ __global__ __launch_bounds__(kBlock) void k_step(
       }
     }
     __syncthreads();
-    const StepGo go = s_go;
-    steps_before = s_steps;
+    StepGo go = s_go;
+    // What comes from LDS is a vector value to the compiler.  The step's
+    // control values are uniform: read into scalar registers they give scalar
+    // branches, not lane masks kept (and spilled) through the whole march.
+    go.stop = __builtin_amdgcn_readfirstlane((int)go.stop) != 0;
+    go.rec_lo = __builtin_amdgcn_readfirstlane(go.rec_lo);
+    go.rec_hi = __builtin_amdgcn_readfirstlane(go.rec_hi);
+    go.ck_slot = __builtin_amdgcn_readfirstlane(go.ck_slot);
+    steps_before = __builtin_amdgcn_readfirstlane(s_steps);
     if (go.stop) {
       // Every workgroup sees the same plan and slots, so all stop here;
       // workgroup 0 accounts the last step taken and marks the loop done.
