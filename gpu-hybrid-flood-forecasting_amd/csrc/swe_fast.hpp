@@ -259,8 +259,8 @@ __host__ __device__ __forceinline__ size_t cls_row_bytes(int nx) {
   return (size_t)((nx + 63) / 64) * kClsBlockBytes;
 }
 
-// The class table of kStaticCls, one per workgroup (filled by the step
-// kernel before the march).
+// The class table of kStaticCls, one per workgroup: dtg n^2 of each class
+// (filled by the step kernel once the step's dt is known, before the march).
 __device__ __forceinline__ float* class_table() {
   __shared__ float t[256];
   return t;
@@ -332,7 +332,7 @@ __device__ __forceinline__ PackedIO packed_io(const Grid& G, const float* S, flo
 // Row j of the lane's column: clamped row index (numerics.py:29-55); the
 // reflective-edge sign flips are applied in side_of.  mc: the uniform
 // multiplier (kStatic8 and kStaticCls).
-template <int LAY>
+template <int LAY, bool EDGE = true>
 __device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, int j, float mc) {
   const int jj = min(max(j, 0), G.ny - 1);
   const unsigned rs = (unsigned)(jj - io.jlo);
@@ -359,7 +359,7 @@ __device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, i
     r.m = mc;
   }
   if constexpr (LAY != kStaticCls) r.a = !signbit(r.n);  // sign bit clear: active
-  r.neg_hv = (j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0);
+  r.neg_hv = EDGE && ((j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0));
   return r;
 }
 
@@ -375,7 +375,7 @@ __device__ __forceinline__ RawCell settled(RawCell r) {
   return r;
 }
 
-template <int LAY>
+template <int LAY, bool EDGE = true>
 __device__ __forceinline__ RawCell load_raw(const Grid& G, const StripIO& io, int j, float) {
   const int jj = min(max(j, 0), G.ny - 1);
   RawCell r;
@@ -384,7 +384,7 @@ __device__ __forceinline__ RawCell load_raw(const Grid& G, const StripIO& io, in
   r.hv = io.ld(io.hv, jj);
   r.b = io.ld(io.bed, jj);
   r.a = io.ld_mask(jj) != 0;
-  r.neg_hv = (j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0);
+  r.neg_hv = EDGE && ((j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0));
   return r;
 }
 
@@ -423,11 +423,13 @@ __device__ __forceinline__ void static_for(F&& f) {
 }
 
 // One wavefront's strip: columns sx*62-1 .. sx*62+62, rows sy*rows ..
-// sy*rows+rows-1.  sx, sy and rows must be wave-uniform.  The raw inputs of
+// sy*rows+rows-1.  sx, sy and rows must be wave-uniform.  EDGE: the strip may
+// touch the grid's edges (ghost columns or rows, open-boundary faces); a
+// strip that does not can take the march without them.  The raw inputs of
 // row j+2 are in flight while row j is computed.  PACKED: H and Ho are packed
 // states (3 floats per cell, HU/HV unused) and the static fields come from
 // G.pk; the arithmetic is the same, so both layouts give identical bits.
-template <bool LOOP, bool PACKED, bool SOUTH, int LAY>
+template <bool LOOP, bool PACKED, bool SOUTH, int LAY, bool EDGE>
 __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
                                                  const float* __restrict__ HV,
@@ -444,12 +446,16 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   const int i = sx * kStripCols - 1 + lane;
   const bool own = lane >= 1 && lane <= kStripCols && i < G.nx;
   const int ic = min(max(i, 0), G.nx - 1);
-  const bool neg_hu = (i < 0 && G.bt[0] == 0) || (i >= G.nx && G.bt[1] == 0);
+  const bool neg_hu = EDGE && ((i < 0 && G.bt[0] == 0) || (i >= G.nx && G.bt[1] == 0));
   // Only the first and last strip of a row band touch the west/east edge faces.
-  const int edge_x = LOOP && (sx == 0 || (sx + 1) * kStripCols >= G.nx);
+  const int edge_x = EDGE && LOOP && (sx == 0 || (sx + 1) * kStripCols >= G.nx);
   const int j0 = sy * rows;
   const int j1 = min(j0 + rows, G.ny);
   const float dry = G.dry;
+  // dry in a vector register too: max(hs, dry) below is a med3 whose other
+  // bound (+inf) is a scalar, and a med3 takes one scalar operand at most.
+  float dry_v = dry;
+  asm("" : "+v"(dry_v));
   using IO = std::conditional_t<PACKED, PackedIO, StripIO>;
   IO io;
   if constexpr (PACKED)
@@ -474,30 +480,30 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   // (in flight, raw[0 ..]).
   constexpr int kSets = kRowsAhead + 1;
   RawCell raw[kSets];
-  RowCell cur = side_of(settled<LAY>(load_raw<LAY>(G, io, jf - dj, mc)), neg_hu, dry, G.half_g);
+  RowCell cur = side_of(settled<LAY>(load_raw<LAY, EDGE>(G, io, jf - dj, mc)), neg_hu, dry, G.half_g);
   float yc0, yc1, yc2;  // carried y face: between the current row and the one behind it
   float nn, mult;
   {
-    const RawCell first = settled<LAY>(load_raw<LAY>(G, io, jf, mc));
+    const RawCell first = settled<LAY>(load_raw<LAY, EDGE>(G, io, jf, mc));
     const RowCell c0 = side_of(first, neg_hu, dry, G.half_g);
     if constexpr (PACKED) {
       nn = first.n;
       mult = first.m;
     }
-    raw[0] = load_raw<LAY>(G, io, jf + dj, mc);
+    raw[0] = load_raw<LAY, EDGE>(G, io, jf + dj, mc);
     static_for<2, kRowsAhead>([&](auto mc_) __attribute__((always_inline)) {
       constexpr int m = decltype(mc_)::value;
-      raw[m - 1] = load_raw<LAY>(G, io, SOUTH ? max(jf - m, j0 - 1) : min(jf + m, j1), mc);
+      raw[m - 1] = load_raw<LAY, EDGE>(G, io, SOUTH ? max(jf - m, j0 - 1) : min(jf + m, j1), mc);
     });
     if constexpr (!SOUTH) {  // south face of row j0
       const FaceFlux fy = face_fast_any<false>(G, cur.s, c0.s, wave_all(cur.s.a && c0.s.a));
-      if (LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
+      if (EDGE && LOOP && j0 == 0 && own) acc.bnd += (double)fy.f0 * dxd;
       yc0 = fy.f0;
       yc1 = fy.ft;
       yc2 = fy.fn + fy.cB;
     } else {  // north face of row j1-1
       const FaceFlux fy = face_fast_any<false>(G, c0.s, cur.s, wave_all(cur.s.a && c0.s.a));
-      if (LOOP && j1 == G.ny && own) acc.bnd -= (double)fy.f0 * dxd;
+      if (EDGE && LOOP && j1 == G.ny && own) acc.bnd -= (double)fy.f0 * dxd;
       yc0 = fy.f0;
       yc1 = fy.ft;
       yc2 = fy.fn + fy.cA;
@@ -519,7 +525,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
                  float nn_j, float mult_j, float& nn_next, float& mult_next) {
     // Unconditional, so the wait counts stay exact; past the strip the
     // clamped rows were just loaded and come from the cache.
-    ahead = load_raw<LAY>(G, io, SOUTH ? max(j - kRowsAhead, j0 - 1) : min(j + kRowsAhead, j1), mc);
+    ahead = load_raw<LAY, EDGE>(G, io, SOUTH ? max(j - kRowsAhead, j0 - 1) : min(j + kRowsAhead, j1), mc);
     const RawCell nrow = settled<LAY>(north);
     if constexpr (PACKED) {  // row j+dj's fields arrived with its state
       nn_next = nrow.n;
@@ -553,9 +559,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
                                  cur.s.eta, cur.s.b, cur.s.u, cur.s.v, cur.s.p, ca);
     }
     const float xr0 = fx.f0, xr1 = fx.fn + fx.cB, xr2 = fx.ft;
-    const float xl0 = from_east(fx.f0);
-    const float xl1 = from_east(fx.fn + fx.cA);
-    const float xl2 = from_east(fx.ft);
+    const float xe1 = fx.fn + fx.cA;  // this lane's share of the face west of lane+1
     // edge_x and the record range are uniform integers tested in every row
     // through an empty asm statement (scalar register, no instruction): a
     // loop-invariant test would be hoisted out of the march as a 64-bit lane
@@ -589,6 +593,11 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     const float yn0 = SOUTH ? yc0 : yf0, yn1 = SOUTH ? yc1 : yf1, yn2 = SOUTH ? yc2 : yf2;  // north
     const float ys0 = SOUTH ? yf0 : yc0, ys1 = SOUTH ? yf1 : yc1, ys2 = SOUTH ? yf2 : yc2;  // south
 
+    // The east faces from lane+1, moved next to their one use (the
+    // subtraction below takes the cross-lane move as its operand).
+    const float xl0 = from_east(fx.f0);
+    const float xl1 = from_east(xe1);
+    const float xl2 = from_east(fx.ft);
     // dt ((xl - xr)/dx + (yn - ys)/dy) as (dt/dx) ((xl - xr) + (yn - ys) dx/dy)
     const float d0 = fmaf(yn0 - ys0, G.rhox, xl0 - xr0);
     const float d1 = fmaf(yn1 - ys1, G.rhox, xl1 - xr1);
@@ -597,13 +606,15 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     const float hus = fmaf(-dtx, d1, cur.hu);
     const float hvs = fmaf(-dtx, d2, cur.hv);
     const float hn = fmaxf(hs, 0.0f);
-    const float hd = max_nc(hs, dry);  // = max(hn, dry): dry > 0
+    const float hd = max_nc(hs, dry_v);  // = max(hn, dry): dry > 0
     const float rhd = __builtin_amdgcn_rcpf(hd);
     const float q = __builtin_amdgcn_sqrtf(fmaf(hus, hus, hvs * hvs));
     // K = dtg n^2 q / hd^(7/3), the power as 2^(-7/3 log2 hd) on the hardware log/exp
     const float ip73 = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(hd) * -kPow73);
     const float n2 = PACKED ? nn_j : nn_j * nn_j;  // the packed field holds n^2
-    const float K = ((dtg * n2) * q) * ip73;
+    // the class table holds dtg n^2 already
+    const float dn2 = LAY == kStaticCls ? n2 : dtg * n2;
+    const float K = (dn2 * q) * ip73;
     const float rD = __builtin_amdgcn_rcpf(1.0f + K);
     const bool wet = ca && (hn > dry);
     const float h_o = ca ? hn : 0.0f;
@@ -649,7 +660,9 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
         const float ws = __builtin_amdgcn_sqrtf(G.g * h_o);
         const float su = fmaf(fabsf(hu_o), rhd, ws);
         const float sv = fmaf(fabsf(hv_o), rhd, ws);
-        acc.inv = max_nc(acc.inv, ca ? fmaf(su, G.rho, sv) : 0.0f);
+        // An inactive cell gives exactly 0 here (h_o, hu_o and hv_o are 0
+        // and rhd is finite), as the lane's maximum starts from.
+        acc.inv = max_nc(acc.inv, fmaf(su, G.rho, sv));
         acc.vol += (double)h_o;
         clip32 += ca ? hn - hs : 0.0f;  // max(-hs, 0), exactly
         // (x + y + z) * 0 stays 0 unless one of them is infinite or NaN
@@ -697,9 +710,9 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   // yc0 now holds the mass flux through the last face of the march: the north
   // face of row j1-1, or (SOUTH) the south face of row j0.
   if constexpr (!SOUTH) {
-    if (LOOP && j1 == G.ny && own) acc.bnd -= (double)yc0 * dxd;
+    if (EDGE && LOOP && j1 == G.ny && own) acc.bnd -= (double)yc0 * dxd;
   } else {
-    if (LOOP && j0 == 0 && own) acc.bnd += (double)yc0 * dxd;
+    if (EDGE && LOOP && j0 == 0 && own) acc.bnd += (double)yc0 * dxd;
   }
   if (LOOP) {
     acc.nf |= !isfinite(nf_probe);
@@ -728,14 +741,31 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
                                                  float* __restrict__ cube_hv, StripAcc& acc,
                                                  float mc = 0.0f) {
   static_assert(PACKED || LAY == kStatic12, "the compact static records are packed layouts");
-  if (sy & 1)
-    march_strip_dir<LOOP, PACKED, true, LAY>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate,
-                                             dtg, dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu,
-                                             cube_hv, acc, mc);
-  else
-    march_strip_dir<LOOP, PACKED, false, LAY>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate,
-                                              dtg, dxd, dyd, rec_lo, rec_hi, cube_h, cube_hu,
-                                              cube_hv, acc, mc);
+  auto march = [&](auto south, auto edge) __attribute__((always_inline)) {
+    march_strip_dir<LOOP, PACKED, decltype(south)::value, LAY, decltype(edge)::value>(
+        G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate, dtg, dxd, dyd, rec_lo, rec_hi,
+        cube_h, cube_hu, cube_hv, acc, mc);
+  };
+  using S = std::true_type;
+  using N = std::false_type;
+  if constexpr (PACKED) {
+    // A strip whose 64 columns and two halo rows all lie inside the grid
+    // (every strip but the first and last of a row band, in every band but
+    // the southmost and northmost) has no reflective-edge sign flips and no
+    // open-boundary faces: it takes the march without that work.
+    const int j0 = sy * rows, j1 = min(j0 + rows, G.ny);
+    const bool interior = sx >= 1 && (sx + 1) * kStripCols < G.nx && j0 >= 1 && j1 < G.ny;
+    if (sy & 1) {
+      if (interior) march(S{}, N{});
+      else march(S{}, S{});
+    } else {
+      if (interior) march(N{}, N{});
+      else march(N{}, S{});
+    }
+  } else {
+    if (sy & 1) march(S{}, S{});
+    else march(N{}, S{});
+  }
 }
 
 }  // namespace swe

@@ -53,11 +53,11 @@ __global__ __launch_bounds__(kBlock) void k_step(
   int rec_lo = 0, rec_hi = 0;
   int steps_before = 0;  // fused loop: index of this step
   int lay = kStatic12;   // packed loop: static record (L.sum_mult[3], set at the loop start)
+  float cls_n2 = 0.0f;   // kStaticCls: n^2 of class threadIdx.x (negative: inactive)
   if constexpr (PACKED) {
     static_assert(kBlock == 256, "one class-table entry per thread");
     lay = (int)L.sum_mult[3];
-    if (lay == kStaticCls) class_table()[threadIdx.x] = G.cls_tab[threadIdx.x];
-    if constexpr (MODE != MODE_LOOP_FUSED) __syncthreads();  // fused: the entry's barrier below
+    if (lay == kStaticCls) cls_n2 = G.cls_tab[threadIdx.x];
   }
   if constexpr (MODE == MODE_LOOP_FUSED) {
     // par: parity of this step.  Every workgroup derives the step from its
@@ -133,6 +133,14 @@ __global__ __launch_bounds__(kBlock) void k_step(
     }
   }
   static_assert(!PACKED || (M::kFast && MODE != MODE_STEP), "packed layout: fast device loop only");
+  if constexpr (PACKED) {
+    // The class table holds dtg n^2, the product the friction factor starts
+    // from (rounded as the march would round it), so the march takes it as
+    // is.  Negative entries (inactive) keep their sign and value.
+    if (lay == kStaticCls)
+      class_table()[threadIdx.x] = signbit(cls_n2) ? cls_n2 : dtg * cls_n2;
+    __syncthreads();
+  }
   int gw = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave_in_block();
   // rev: this step sweeps the strips in the opposite order, so that its
   // first strips are the ones the previous step touched last and still has
