@@ -318,17 +318,28 @@ __device__ __forceinline__ int wave_in_block() {
 }
 
 // Wavefront reductions (all lanes receive the result).
+//
+// Maximum of non-negative floats (the CFL inverses; no NaN), taken on their
+// bit patterns, which order as the values do: DPP row shifts by 1, 2, 4 and 8
+// leave each 16-lane row's maximum in its lane 15, two row broadcasts carry
+// it to lane 63.  A lane without a source reads 0, the identity here.
 __device__ __forceinline__ float wave_max(float x) {
-  for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, kWave));
-  return x;
+  unsigned u = __float_as_uint(x);
+  u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x111, 0xF, 0xF, true));  // row_shr:1
+  u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x112, 0xF, 0xF, true));  // row_shr:2
+  u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x114, 0xF, 0xF, true));  // row_shr:4
+  u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x118, 0xF, 0xF, true));  // row_shr:8
+  u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x142, 0xA, 0xF, false));  // row_bcast:15
+  u = max(u, (unsigned)__builtin_amdgcn_update_dpp(0, (int)u, 0x143, 0xC, 0xF, false));  // row_bcast:31
+  return __uint_as_float(__builtin_amdgcn_readlane(u, kWave - 1));
 }
 __device__ __forceinline__ double wave_sum(double x) {
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
   return x;
 }
-__device__ __forceinline__ int wave_or(int x) {
-  for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, kWave);
-  return x;
+// 1 if x is nonzero on any active lane, else 0 (the flags reduced here are 0 or 1).
+__device__ __forceinline__ int wave_any(int x) {
+  return __builtin_amdgcn_ballot_w64(x != 0) != 0 ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
     const double vol = wave_sum(acc.vol);
     const double clip = wave_sum(acc.clip);
     const double bnd = wave_sum(acc.bnd);
-    const int nf = wave_or(acc.nf);
+    const int nf = wave_any(acc.nf);
     if (lane == 0) {
       P.inv[w] = inv;
       P.vol[w] = vol;

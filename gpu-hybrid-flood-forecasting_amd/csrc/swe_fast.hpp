@@ -429,7 +429,7 @@ __device__ __forceinline__ void static_for(F&& f) {
 // row j+2 are in flight while row j is computed.  PACKED: H and Ho are packed
 // states (3 floats per cell, HU/HV unused) and the static fields come from
 // G.pk; the arithmetic is the same, so both layouts give identical bits.
-template <bool LOOP, bool PACKED, bool SOUTH, int LAY, bool EDGE>
+template <bool LOOP, bool PACKED, bool SOUTH, int LAY, bool EDGE, bool VOL>
 __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
                                                  const float* __restrict__ HV,
@@ -663,7 +663,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
         // An inactive cell gives exactly 0 here (h_o, hu_o and hv_o are 0
         // and rhd is finite), as the lane's maximum starts from.
         acc.inv = max_nc(acc.inv, fmaf(su, G.rho, sv));
-        acc.vol += (double)h_o;
+        if constexpr (VOL) acc.vol += (double)h_o;
         clip32 += ca ? hn - hs : 0.0f;  // max(-hs, 0), exactly
         // (x + y + z) * 0 stays 0 unless one of them is infinite or NaN
         nf_probe = fmaf(h_o + hu_o + hv_o, 0.0f, nf_probe);
@@ -728,7 +728,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
 // the second read is an L2 hit.  The direction does not change any cell's
 // arithmetic: every face takes the same operands, and a cell combines its
 // faces in the same order (north minus south, east minus west).
-template <bool LOOP, bool PACKED = false, int LAY = kStatic12>
+template <bool LOOP, bool PACKED = false, int LAY = kStatic12, bool VOL = true>
 __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
                                                  const float* __restrict__ HV,
@@ -742,7 +742,7 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
                                                  float mc = 0.0f) {
   static_assert(PACKED || LAY == kStatic12, "the compact static records are packed layouts");
   auto march = [&](auto south, auto edge) __attribute__((always_inline)) {
-    march_strip_dir<LOOP, PACKED, decltype(south)::value, LAY, decltype(edge)::value>(
+    march_strip_dir<LOOP, PACKED, decltype(south)::value, LAY, decltype(edge)::value, VOL>(
         G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate, dtg, dxd, dyd, rec_lo, rec_hi,
         cube_h, cube_hu, cube_hv, acc, mc);
   };

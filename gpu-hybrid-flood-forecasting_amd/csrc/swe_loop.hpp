@@ -158,10 +158,13 @@ __device__ __forceinline__ Totals block_reduce(float inv, double vol, double cli
   __shared__ double s_vol[WAVES], s_clip[WAVES], s_bnd[WAVES];
   __shared__ int s_nf[WAVES];
   inv = wave_max(inv);
-  vol = wave_sum(vol);
-  clip = wave_sum(clip);
-  bnd = wave_sum(bnd);
-  nf = wave_or(nf);
+  // Sums that are zero on every lane (the clip and open-boundary volumes of
+  // most wavefronts, the volume where the caller does not sum it) skip the
+  // shuffles: a uniform branch on one comparison.
+  if (__builtin_amdgcn_ballot_w64(vol != 0.0) != 0) vol = wave_sum(vol);
+  if (__builtin_amdgcn_ballot_w64(clip != 0.0) != 0) clip = wave_sum(clip);
+  if (__builtin_amdgcn_ballot_w64(bnd != 0.0) != 0) bnd = wave_sum(bnd);
+  nf = wave_any(nf);
   const int w = threadIdx.x / kWave;
   if ((threadIdx.x & (kWave - 1)) == 0) {
     s_inv[w] = inv;

@@ -13,6 +13,7 @@
 //                      output slots and stop flag (numerics.py:398-418).
 //   k_reduce<M, MODE>  CFL maximum + volume of a state (numerics.py:297-309);
 //                      MODE_INIT also primes the loop control block.
+//   k_volume_packed    volume of a packed loop's final state.
 //   k_cfl_field<M>     per-cell inverse CFL time.
 //   k_sum_mult         FP64 sum of the rain multiplier over active cells.
 
@@ -154,16 +155,16 @@ __global__ __launch_bounds__(kBlock) void k_step(
       // is L.sum_mult[2].
       const float mc = (float)L.sum_mult[2];
       if (lay == kStaticCls)
-        march_strip_fast<true, true, kStaticCls>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
+        march_strip_fast<true, true, kStaticCls, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
                                                  gw / segs_x, rows, dtx, dtrate, dtg, L.dxd,
                                                  L.dyd, rec_lo, rec_hi, cube_h, cube_hu, cube_hv,
                                                  acc, mc);
       else if (lay == kStatic8)
-        march_strip_fast<true, true, kStatic8>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
+        march_strip_fast<true, true, kStatic8, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
                                                gw / segs_x, rows, dtx, dtrate, dtg, L.dxd, L.dyd,
                                                rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc, mc);
       else
-        march_strip_fast<true, true, kStatic12>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
+        march_strip_fast<true, true, kStatic12, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
                                                 gw / segs_x, rows, dtx, dtrate, dtg, L.dxd, L.dyd,
                                                 rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
     } else if constexpr (M::kFast)
@@ -256,6 +257,30 @@ __global__ __launch_bounds__(kBlock) void k_close(LoopControl* __restrict__ ctl,
   account_before<kFinishUnroll>(ctl, L, P, (1 - p) * P.stride, nparts, pl,
                                 __uint_as_float(nonfinite ? 0u : bits), nonfinite);
   if (threadIdx.x == 0 && (pl.done || nonfinite)) ctl->done = 1;
+}
+
+// Volume of the final state of a packed loop, whose step kernels do not sum
+// it: the active cells' depths in float64, with the cell-to-thread mapping,
+// launch shape and partial sums of k_reduce, so that the final volume is
+// added up in the same order as the initial one.  The state is work buffer a
+// or b by the parity of the loop's step count.  Sets ctl->vol.
+__global__ __launch_bounds__(kBlock) void k_volume_packed(Grid G, const float* __restrict__ a,
+                                                         const float* __restrict__ b,
+                                                         LoopControl* __restrict__ ctl,
+                                                         LoopConst L, Partials P) {
+  const float* S = (ctl->steps & 1) ? b : a;
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  double vol = 0.0;
+  for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < N;
+       k += (size_t)gridDim.x * kBlock)
+    vol += G.mask[k] != 0 ? (double)S[3 * k] : 0.0;
+  const Totals mine = block_reduce(0.0f, vol, 0.0, 0.0, 0);
+  if (!publish_partials(P, ctl, mine)) return;
+  const Totals tot = gather_partials(P, gridDim.x);
+  if (threadIdx.x == 0) {
+    ctl->vol = tot.vol * L.cell_area;
+    __hip_atomic_store(&ctl->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 template <class M, int MODE>
@@ -964,12 +989,16 @@ int launch_loop_chunk(SweContext* c, int count, const LoopConst& L, float* ch, f
   return SWE_OK;
 }
 
-// After the last step kernel of a fused loop: account the last step taken
-// (k_close); the control block then holds the loop's final clock, step count,
-// output cursor and FP64 balance, as after a finish kernel.
+// After the last step kernel of a loop.  Fused loop: account the last step
+// taken (k_close); the control block then holds the loop's final clock, step
+// count, output cursor and FP64 balance, as after a finish kernel.  Packed
+// loop: sum the final state's volume (k_volume_packed).
 int close_loop(SweContext* c, const LoopConst& L, hipStream_t s) {
-  if (c->split_finish != 1) return SWE_OK;
-  hipLaunchKernelGGL(k_close, dim3(1), dim3(kBlock), 0, s, c->d_ctl, L, c->P, c->step_blocks());
+  if (c->split_finish == 1)
+    hipLaunchKernelGGL(k_close, dim3(1), dim3(kBlock), 0, s, c->d_ctl, L, c->P, c->step_blocks());
+  if (c->loop_packed)
+    hipLaunchKernelGGL(k_volume_packed, dim3(c->reduce_blocks()), dim3(kBlock), 0, s, c->G,
+                       c->ws(0), c->ws(1), c->d_ctl, L, c->P);
   SWE_HIP_TRY(hipGetLastError());
   return SWE_OK;
 }
