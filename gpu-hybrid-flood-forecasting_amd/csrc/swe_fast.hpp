@@ -514,7 +514,11 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     nn = io.ld(io.n, jf);
     mult = io.ld(io.mult, jf);
   }
-  float nf_probe = 0.0f;
+  // Lane maximum of the CFL terms below, taken on their bit patterns: the
+  // terms are +0 or positive, which order as their patterns do, and a
+  // non-finite new h, hu or hv makes its term infinite or NaN, whose pattern
+  // (either sign) is at least +inf's, so the same maximum flags it.
+  unsigned inv_bits = 0u;
   float clip32 = 0.0f;  // clip depth is small; float32 per lane over the strip
 
   // Row j: `cur` is row j, `north` row j+dj (the next row of the march,
@@ -660,13 +664,11 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
         const float ws = __builtin_amdgcn_sqrtf(G.g * h_o);
         const float su = fmaf(fabsf(hu_o), rhd, ws);
         const float sv = fmaf(fabsf(hv_o), rhd, ws);
-        // An inactive cell gives exactly 0 here (h_o, hu_o and hv_o are 0
-        // and rhd is finite), as the lane's maximum starts from.
-        acc.inv = max_nc(acc.inv, fmaf(su, G.rho, sv));
+        // An inactive cell gives exactly +0 here (h_o, hu_o and hv_o are 0
+        // and rhd is finite: hd >= dry > 0), as the lane's maximum starts from.
+        inv_bits = max(inv_bits, __float_as_uint(fmaf(su, G.rho, sv)));
         if constexpr (VOL) acc.vol += (double)h_o;
         clip32 += ca ? hn - hs : 0.0f;  // max(-hs, 0), exactly
-        // (x + y + z) * 0 stays 0 unless one of them is infinite or NaN
-        nf_probe = fmaf(h_o + hu_o + hv_o, 0.0f, nf_probe);
       }
     }
     yc0 = gy.f0;
@@ -715,9 +717,11 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     if (EDGE && LOOP && j0 == 0 && own) acc.bnd += (double)yc0 * dxd;
   }
   if (LOOP) {
-    acc.nf |= !isfinite(nf_probe);
+    const bool nonfinite = inv_bits >= 0x7F800000u;  // +inf, NaN or a negative pattern
+    acc.nf |= nonfinite;
     acc.clip += (double)clip32;
-    acc.inv *= G.rdy;  // max(x) / dy = max(x / dy): rounding is monotonic
+    // max(x) / dy = max(x / dy): rounding is monotonic
+    acc.inv = nonfinite ? 0.0f : __uint_as_float(inv_bits) * G.rdy;
   }
 }
 

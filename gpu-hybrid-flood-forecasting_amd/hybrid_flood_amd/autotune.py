@@ -31,8 +31,8 @@ CANDIDATE_ROWS = (4, 6, 8, 12, 14, 16, 17, 20, 21, 24, 32, 48, 64)
 # Resident wavefronts per compute unit: 0 = as many as registers allow.
 CANDIDATE_WAVES = (0, 20, 16, 12)
 # Resident wavefronts per compute unit of the fused loop step when registers
-# alone limit it (85 VGPRs: 5 per SIMD).
-UNCAPPED_WAVES = 20
+# alone limit it (79 VGPRs: 6 per SIMD).
+UNCAPPED_WAVES = 24
 STRIP_COLS = 62
 
 
