@@ -73,8 +73,17 @@ __device__ __forceinline__ int find_member(const Member* __restrict__ mem, int n
   return lo;
 }
 
+// One class table per wavefront of a workgroup (members differ between the
+// wavefronts of one workgroup).
+__device__ __forceinline__ float* ens_class_tables() {
+  __shared__ float t[kWavesPerBlock * 256];
+  return t;
+}
+
 // PACKED (fast variant): members' work states and static fields in the packed
-// layout of the single-domain fast loop (swe_fast.hpp), same arithmetic.
+// layout of the single-domain fast loop (swe_fast.hpp), with the static record
+// each member's start chose (the class-indexed one, the (bed, n') one or the
+// full one), same arithmetic.
 template <class M, bool PACKED = false>
 __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict__ mem,
                                                       int n_members, int total_waves, int rev,
@@ -108,11 +117,43 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
     const float* src = mb.buf[par];
     float* dst = mb.buf[par ^ 1];
     StripAcc acc;
-    if constexpr (M::kFast)
-      march_strip_fast<true, PACKED>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
-                             local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dtx_f,
-                             ctl->dtrate_f, ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr,
-                             nullptr, nullptr, acc);
+    if constexpr (M::kFast && PACKED) {
+      // The member's static record, chosen at its start (swe_ensemble_begin):
+      // sum_mult[3] is the record, sum_mult[2] the uniform multiplier.
+      const int lay = __builtin_amdgcn_readfirstlane((int)mb.L.sum_mult[3]);
+      const float mc = (float)mb.L.sum_mult[2];
+      const float dtg = ctl->dtg_f;
+      const int sx = local % mb.segs_x, sy = local / mb.segs_x;
+      if (lay == kStaticCls) {
+        // This wavefront's class table: dtg n^2 of the member's classes,
+        // rounded as the single-domain step rounds them.  The wavefront's
+        // own LDS accesses complete in order, so the march reads what these
+        // stores wrote, and a later strip's refill follows its reads.
+        float* tab = ens_class_tables() + wave_in_block() * 256;
+        for (int k = lane; k < 256; k += kWave) {
+          const float v = mb.G.cls_tab[k];
+          tab[k] = signbit(v) ? v : dtg * v;
+        }
+        march_strip_fast<true, true, kStaticCls>(mb.G, src, nullptr, nullptr, dst, nullptr,
+                                                 nullptr, sx, sy, mb.rows, ctl->dtx_f,
+                                                 ctl->dtrate_f, dtg, mb.L.dxd, mb.L.dyd, 0, 0,
+                                                 nullptr, nullptr, nullptr, acc, mc, tab);
+      } else if (lay == kStatic8) {
+        march_strip_fast<true, true, kStatic8>(mb.G, src, nullptr, nullptr, dst, nullptr, nullptr,
+                                               sx, sy, mb.rows, ctl->dtx_f, ctl->dtrate_f, dtg,
+                                               mb.L.dxd, mb.L.dyd, 0, 0, nullptr, nullptr, nullptr,
+                                               acc, mc);
+      } else {
+        march_strip_fast<true, true, kStatic12>(mb.G, src, nullptr, nullptr, dst, nullptr,
+                                                nullptr, sx, sy, mb.rows, ctl->dtx_f,
+                                                ctl->dtrate_f, dtg, mb.L.dxd, mb.L.dyd, 0, 0,
+                                                nullptr, nullptr, nullptr, acc);
+      }
+    } else if constexpr (M::kFast)
+      march_strip_fast<true, false>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
+                                    local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dtx_f,
+                                    ctl->dtrate_f, ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr,
+                                    nullptr, nullptr, acc);
     else
       march_strip<M, true>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
                            local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dt_f, ctl->dtrate_f,
@@ -172,7 +213,10 @@ struct SweEnsemble {
   double* d_rain = nullptr;      // all members' series
   double* d_times = nullptr;     // per member {0, t_end}
   float* d_work = nullptr;       // all members' A|B states
-  float* d_pk = nullptr;         // fast variant: all members' packed static fields (3 per cell)
+  char* d_pk = nullptr;          // fast variant: all members' packed static records
+  char* d_cls = nullptr;         // fast variant: each member's Manning classes (class_buf_bytes)
+  double* d_sm = nullptr;        // per member: multiplier sum, uniform flag and value, record
+  double* d_scratch = nullptr;   // 2 x kSumBlocks: scratch of the multiplier sums
   EnsPartials P{};
   int parts_cap = 0;
   int total_waves = 0;
@@ -186,6 +230,10 @@ struct SweEnsemble {
 };
 
 namespace {
+
+// Blocks of the multiplier sum: as a single-domain context's reduction
+// (at most 2048), so a member's rain volume has the same bits as its solo run.
+constexpr int kSumBlocks = 2048;
 
 // Lay out the strips of the members flagged in `running` (all when null)
 // into the device member table.  When few cells remain, strips are made
@@ -305,21 +353,16 @@ __global__ __launch_bounds__(256) void k_ens_load(const Member* __restrict__ mem
   const size_t N = (size_t)G.nx * (size_t)G.ny;
   float* A = mb.buf[0];
   float inv = 0.0f;
-  double vol = 0.0, msum = 0.0;
+  double vol = 0.0;
   int nf = 0;
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < N;
        k += (size_t)gridDim.x * blockDim.x) {
     const float a0 = h[k], a1 = hu[k], a2 = hv[k];
     const int a = G.mask[k] != 0;
-    msum += a ? (double)G.mult[k] : 0.0;
     if (G.pk) {
       A[3 * k] = a0;
       A[3 * k + 1] = a1;
       A[3 * k + 2] = a2;
-      float* pk = const_cast<float*>(G.pk);
-      pk[3 * k] = G.bed[k];
-      pk[3 * k + 1] = a ? G.n[k] * G.n[k] : -1.0f;
-      pk[3 * k + 2] = G.mult[k];
     } else {
       A[k] = a0;
       A[N + k] = a1;
@@ -329,15 +372,11 @@ __global__ __launch_bounds__(256) void k_ens_load(const Member* __restrict__ mem
     vol += a ? (double)a0 : 0.0;
     nf |= !(isfinite(a0) && isfinite(a1) && isfinite(a2));
   }
-  // The multiplier's FP64 sum over active cells rides in the clip partial
-  // (unused at the start): the member's fields may have changed in place
-  // since the last run, and its rain volume must follow them.
-  const Totals tot = block_reduce(inv, vol, msum, 0.0, nf);
+  const Totals tot = block_reduce(inv, vol, 0.0, 0.0, nf);
   if (threadIdx.x == 0) {
     const int q = mb.wave_base + blockIdx.x;
     P.inv[q] = tot.inv;
     P.vol[q] = tot.vol;
-    P.clip[q] = tot.clip;
     P.nf[q] = tot.nf;
   }
 }
@@ -348,16 +387,15 @@ __global__ __launch_bounds__(256) void k_ens_init(const Member* __restrict__ mem
                                                   int parts, EnsPartials P) {
   const Member& mb = mem[m];
   float inv = 0.0f;
-  double vol = 0.0, msum = 0.0;
+  double vol = 0.0;
   int nf = 0;
   for (int k = threadIdx.x; k < parts; k += blockDim.x) {
     const int q = mb.wave_base + k;
     inv = fmaxf(inv, P.inv[q]);
     vol += P.vol[q];
-    msum += P.clip[q];
     nf |= P.nf[q];
   }
-  const Totals tot = block_reduce(inv, vol, msum, 0.0, nf);
+  const Totals tot = block_reduce(inv, vol, 0.0, 0.0, nf);
   const double t0 = mb.L.times[0];
   __shared__ int s_j;
   if (threadIdx.x == 0) s_j = 0;
@@ -366,7 +404,6 @@ __global__ __launch_bounds__(256) void k_ens_init(const Member* __restrict__ mem
     if (mb.L.rain_t[k] <= t0 && t0 < mb.L.rain_t[k + 1]) s_j = k;
   __syncthreads();
   if (threadIdx.x == 0) {
-    *const_cast<double*>(mb.L.sum_mult) = tot.clip;  // read by this run's finishes
     LoopControl* ctl = mb.ctl;
     ctl->t = t0;
     ctl->dt = 0.0;
@@ -435,25 +472,37 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
     swe_ensemble_destroy(e);
     return rc;
   };
-  size_t rain_total = 0, cells_total = 0;
+  size_t rain_total = 0, cells_total = 0, pk_total = 0;
+  // A member's packed static record takes at most 12 bytes per cell, or the
+  // class blocks of a narrow grid; each starts on a 256-byte boundary.
+  auto pk_bytes = [](const SweParamsDesc& p) {
+    const size_t b = std::max((size_t)12 * p.nx * p.ny, cls_row_bytes(p.nx) * (size_t)p.ny);
+    return (b + 255) / 256 * 256;
+  };
   for (int m = 0; m < n; ++m) {
     rain_total += 2 * (size_t)members[m].params.n_rainfall;
     cells_total += (size_t)members[m].params.nx * members[m].params.ny;
+    pk_total += pk_bytes(members[m].params);
   }
-  // d_rain holds every member's rain series, then one FP64 multiplier sum per member.
-  if (hipMalloc(&e->d_rain, sizeof(double) * (rain_total + n)) != hipSuccess ||
+  const size_t cls_bytes = (class_buf_bytes() + 255) / 256 * 256;
+  if (hipMalloc(&e->d_rain, sizeof(double) * rain_total) != hipSuccess ||
+      hipMalloc(&e->d_sm, sizeof(double) * 4 * n) != hipSuccess ||
+      hipMalloc(&e->d_scratch, sizeof(double) * 2 * kSumBlocks) != hipSuccess ||
       hipMalloc(&e->d_times, sizeof(double) * 2 * n) != hipSuccess ||
       hipMalloc(&e->d_ctl, sizeof(LoopControl) * n) != hipSuccess ||
       hipHostMalloc(&e->h_ctl, sizeof(LoopControl) * n, hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&e->d_work, sizeof(float) * 6 * cells_total) != hipSuccess ||
-      (variant == SWE_VARIANT_FAST && hipMalloc(&e->d_pk, sizeof(float) * 3 * cells_total) != hipSuccess) ||
+      (variant == SWE_VARIANT_FAST && (hipMalloc(&e->d_pk, pk_total) != hipSuccess ||
+                                       hipMalloc(&e->d_cls, cls_bytes * n) != hipSuccess)) ||
       hipMalloc(&e->d_mem, sizeof(Member) * n) != hipSuccess)
     return bail(efail(SWE_EHIP, "swe_ensemble_create: device allocation failed"));
+  if (hipMemset(e->d_sm, 0, sizeof(double) * 4 * n) != hipSuccess)
+    return bail(efail(SWE_EHIP, "swe_ensemble_create: clear the record slots"));
   if (hipMemset(e->d_ctl, 0, sizeof(LoopControl) * n) != hipSuccess)
     return bail(efail(SWE_EHIP, "swe_ensemble_create: clear control"));
   std::vector<double> rain_all;
   rain_all.reserve(rain_total);
-  size_t rain_off = 0, cell_off = 0;
+  size_t rain_off = 0, cell_off = 0, pk_off = 0;
   for (int m = 0; m < n; ++m) {
     const SweParamsDesc& p = members[m].params;
     Member& mb = e->host[m];
@@ -495,14 +544,17 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
     const size_t N = (size_t)p.nx * p.ny;
     mb.buf[0] = e->d_work + cell_off;
     mb.buf[1] = e->d_work + cell_off + 3 * N;
-    G.pk = e->d_pk ? e->d_pk + cell_off / 2 : nullptr;  // 3 floats per cell
+    G.pk = e->d_pk ? reinterpret_cast<const float*>(e->d_pk + pk_off) : nullptr;
+    G.cls_tab = e->d_cls ? class_values(e->d_cls + cls_bytes * m) : nullptr;
+    pk_off += pk_bytes(p);
     cell_off += 6 * N;
+    L.sum_mult = e->d_sm + 4 * m;
   }
   if (hipMemcpy(e->d_rain, rain_all.data(), sizeof(double) * rain_all.size(), hipMemcpyHostToDevice) != hipSuccess)
     return bail(efail(SWE_EHIP, "swe_ensemble_create: upload rain"));
-  // Each member's FP64 rain-multiplier sum over active cells lives on the
-  // device; swe_ensemble_begin takes it from the member's current arrays.
-  for (int m = 0; m < n; ++m) e->host[m].L.sum_mult = e->d_rain + rain_total + m;
+  // Each member's FP64 rain-multiplier sum over active cells and its static
+  // record live on the device; swe_ensemble_begin takes them from the
+  // member's current arrays.
   int rc = layout(e);
   if (rc) return bail(rc);
   *out = e;
@@ -517,6 +569,9 @@ int swe_ensemble_destroy(SweEnsemble* e) {
   if (e->d_mem) (void)hipFree(e->d_mem);
   if (e->d_work) (void)hipFree(e->d_work);
   if (e->d_pk) (void)hipFree(e->d_pk);
+  if (e->d_cls) (void)hipFree(e->d_cls);
+  if (e->d_sm) (void)hipFree(e->d_sm);
+  if (e->d_scratch) (void)hipFree(e->d_scratch);
   if (e->h_ctl) (void)hipHostFree(e->h_ctl);
   if (e->d_ctl) (void)hipFree(e->d_ctl);
   if (e->d_times) (void)hipFree(e->d_times);
@@ -557,9 +612,19 @@ int swe_ensemble_begin(SweEnsemble* e, double t_end, int32_t max_steps, void* st
   ENS_TRY(hipMemcpy(e->d_times, times.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
   int lrc = layout(e);  // every member, in order: table index m == member m
   if (lrc) return lrc;
+  const size_t cls_bytes = (class_buf_bytes() + 255) / 256 * 256;
   for (int m = 0; m < n; ++m) {
     const SweState& st = e->desc[m].state;
     const size_t N = (size_t)e->host[m].G.nx * e->host[m].G.ny;
+    // The member's multiplier sum (its fields may have changed in place since
+    // the last run, and its rain volume must follow them) and, in the fast
+    // variant, its static record: the same passes, with the same block count,
+    // as a single-domain loop start on the member's grid.
+    int prc = prepare_static_record(e->host[m].G, e->d_cls ? e->d_cls + cls_bytes * m : nullptr,
+                                    e->d_sm + 4 * m, 1, e->d_scratch,
+                                    reinterpret_cast<unsigned*>(e->d_scratch + kSumBlocks),
+                                    kSumBlocks, s);
+    if (prc) return prc;
     // Workgroups of the load pass: ~8 cells per thread, at most one per
     // partial slot the member owns.
     const int nb = (int)std::max<size_t>(
@@ -672,6 +737,17 @@ int swe_ensemble_run(SweEnsemble* e, double t_end, int32_t max_steps, SweRunInfo
       return efail(SWE_EHIP, "swe_ensemble_run: members did not stop at max_steps");
   }
   return swe_ensemble_end(e, infos, stream);
+}
+
+int swe_ensemble_get_static_layout(SweEnsemble* e, int32_t member, int32_t* layout) {
+  if (!e || !layout || member < 0 || member >= (int32_t)e->host.size())
+    return efail(SWE_EINVAL, "swe_ensemble_get_static_layout: bad arguments");
+  // the member's start may have been queued on any stream of the device
+  ENS_TRY(hipDeviceSynchronize());
+  double v = 0.0;
+  ENS_TRY(hipMemcpy(&v, e->d_sm + 4 * member + 3, sizeof(double), hipMemcpyDeviceToHost));
+  *layout = (int32_t)v;
+  return SWE_OK;
 }
 
 int swe_ensemble_total_waves(const SweEnsemble* e, int32_t* waves, int32_t* grid) {

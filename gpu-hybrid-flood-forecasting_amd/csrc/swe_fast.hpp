@@ -259,8 +259,11 @@ __host__ __device__ __forceinline__ size_t cls_row_bytes(int nx) {
   return (size_t)((nx + 63) / 64) * kClsBlockBytes;
 }
 
-// The class table of kStaticCls, one per workgroup: dtg n^2 of each class
-// (filled by the step kernel once the step's dt is known, before the march).
+// The class table of kStaticCls of the single-domain step, one per
+// workgroup: dtg n^2 of each class (filled by the step kernel once the step's
+// dt is known, before the march).  The march reads whatever table it is given
+// (PackedIO::tab), so a kernel whose wavefronts step different domains gives
+// each wavefront its own.
 __device__ __forceinline__ float* class_table() {
   __shared__ float t[256];
   return t;
@@ -278,6 +281,7 @@ struct PackedIO {
   unsigned row_st;   // bytes per packed state row (12 nx)
   unsigned row_sx;   // bytes per packed static row (12 nx, 8 nx, or 320 ceil(nx / 64))
   int jlo;           // row the resources start at
+  const float* tab;  // kStaticCls: the class table in LDS (dtg n^2, or -1 for class 255)
 };
 
 // A wave-uniform pointer, read from lane 0 so that it sits in scalar
@@ -304,9 +308,10 @@ typedef unsigned U32x3 __attribute__((ext_vector_type(3)));
 
 template <int LAY>
 __device__ __forceinline__ PackedIO packed_io(const Grid& G, const float* S, float* O, int ic,
-                                              int jlo) {
+                                              int jlo, const float* tab) {
   PackedIO io;
   io.jlo = jlo;
+  io.tab = tab;
   io.row_st = 12u * (unsigned)G.nx;
   const size_t rows = (size_t)(G.ny - jlo);
   const size_t st0 = (size_t)jlo * io.row_st;
@@ -366,10 +371,10 @@ __device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, i
 // The row's class looked up: n' and the active flag as the 8-byte record
 // gives them.  Called where the row is first used, so the loads of the rows
 // behind it stay in flight.
-template <int LAY>
-__device__ __forceinline__ RawCell settled(RawCell r) {
+template <int LAY, class IO>
+__device__ __forceinline__ RawCell settled(RawCell r, const IO& io) {
   if constexpr (LAY == kStaticCls) {
-    r.n = class_table()[r.c];
+    r.n = io.tab[r.c];
     r.a = !signbit(r.n);
   }
   return r;
@@ -440,7 +445,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
                                                  float* __restrict__ cube_h,
                                                  float* __restrict__ cube_hu,
                                                  float* __restrict__ cube_hv, StripAcc& acc,
-                                                 float mc) {
+                                                 float mc, const float* tab) {
   const size_t N = (size_t)G.nx * (size_t)G.ny;
   const int lane = threadIdx.x & (kWave - 1);
   const int i = sx * kStripCols - 1 + lane;
@@ -459,7 +464,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   using IO = std::conditional_t<PACKED, PackedIO, StripIO>;
   IO io;
   if constexpr (PACKED)
-    io = packed_io<LAY>(G, H, Ho, ic, max(j0 - 1, 0));
+    io = packed_io<LAY>(G, H, Ho, ic, max(j0 - 1, 0), tab);
   else
     io = strip_io(G, H, HU, HV, max(j0 - 1, 0), ic);
   const size_t out0 = (size_t)j0 * (size_t)G.nx, out_cells = (size_t)(j1 - j0) * (size_t)G.nx;
@@ -480,11 +485,11 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   // (in flight, raw[0 ..]).
   constexpr int kSets = kRowsAhead + 1;
   RawCell raw[kSets];
-  RowCell cur = side_of(settled<LAY>(load_raw<LAY, EDGE>(G, io, jf - dj, mc)), neg_hu, dry, G.half_g);
+  RowCell cur = side_of(settled<LAY>(load_raw<LAY, EDGE>(G, io, jf - dj, mc), io), neg_hu, dry, G.half_g);
   float yc0, yc1, yc2;  // carried y face: between the current row and the one behind it
   float nn, mult;
   {
-    const RawCell first = settled<LAY>(load_raw<LAY, EDGE>(G, io, jf, mc));
+    const RawCell first = settled<LAY>(load_raw<LAY, EDGE>(G, io, jf, mc), io);
     const RowCell c0 = side_of(first, neg_hu, dry, G.half_g);
     if constexpr (PACKED) {
       nn = first.n;
@@ -530,7 +535,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     // Unconditional, so the wait counts stay exact; past the strip the
     // clamped rows were just loaded and come from the cache.
     ahead = load_raw<LAY, EDGE>(G, io, SOUTH ? max(j - kRowsAhead, j0 - 1) : min(j + kRowsAhead, j1), mc);
-    const RawCell nrow = settled<LAY>(north);
+    const RawCell nrow = settled<LAY>(north, io);
     if constexpr (PACKED) {  // row j+dj's fields arrived with its state
       nn_next = nrow.n;
       mult_next = nrow.m;
@@ -743,12 +748,12 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
                                                  float* __restrict__ cube_h,
                                                  float* __restrict__ cube_hu,
                                                  float* __restrict__ cube_hv, StripAcc& acc,
-                                                 float mc = 0.0f) {
+                                                 float mc = 0.0f, const float* tab = nullptr) {
   static_assert(PACKED || LAY == kStatic12, "the compact static records are packed layouts");
   auto march = [&](auto south, auto edge) __attribute__((always_inline)) {
     march_strip_dir<LOOP, PACKED, decltype(south)::value, LAY, decltype(edge)::value, VOL>(
         G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate, dtg, dxd, dyd, rec_lo, rec_hi,
-        cube_h, cube_hu, cube_hv, acc, mc);
+        cube_h, cube_hu, cube_hv, acc, mc, tab);
   };
   using S = std::true_type;
   using N = std::false_type;

@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
         march_strip_fast<true, true, kStaticCls, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
                                                  gw / segs_x, rows, dtx, dtrate, dtg, L.dxd,
                                                  L.dyd, rec_lo, rec_hi, cube_h, cube_hu, cube_hv,
-                                                 acc, mc);
+                                                 acc, mc, class_table());
       else if (lay == kStatic8)
         march_strip_fast<true, true, kStatic8, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
                                                gw / segs_x, rows, dtx, dtrate, dtg, L.dxd, L.dyd,
@@ -612,6 +612,47 @@ __global__ __launch_bounds__(kBlock) void k_copy_final(const LoopControl* __rest
   }
 }
 
+// ---- The static record of a packed loop, for any grid ------------------------
+size_t class_buf_bytes() {
+  return sizeof(unsigned long long) * kClsSlots + sizeof(float) * 256 + kClsSlots + sizeof(int);
+}
+
+const float* class_values(const void* cls_mem) {
+  return reinterpret_cast<const float*>(static_cast<const char*>(cls_mem) +
+                                        sizeof(unsigned long long) * kClsSlots);
+}
+
+ClassBuf class_buf_at(void* mem) {
+  char* m = static_cast<char*>(mem);
+  constexpr size_t kKeys = sizeof(unsigned long long) * kClsSlots, kVals = sizeof(float) * 256;
+  return ClassBuf{reinterpret_cast<unsigned long long*>(m), reinterpret_cast<float*>(m + kKeys),
+                  reinterpret_cast<unsigned char*>(m + kKeys + kVals),
+                  reinterpret_cast<int*>(m + kKeys + kVals + kClsSlots)};
+}
+
+int prepare_static_record(const Grid& G, void* cls_mem, double* sm, int allow_classes,
+                          double* sum_part, unsigned* bits_part, int max_blocks, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const size_t N = (size_t)G.nx * (size_t)G.ny;
+  const int nb = (int)std::max<size_t>(
+      1, std::min<size_t>((N + (size_t)kBlock * 8 - 1) / ((size_t)kBlock * 8), (size_t)max_blocks));
+  hipLaunchKernelGGL(k_sum_mult, dim3(nb), dim3(kBlock), 0, s, G, sum_part, bits_part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, sum_part, bits_part, nb, sm);
+  if (cls_mem) {
+    const ClassBuf B = class_buf_at(cls_mem);
+    const int np = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_cls_clear, dim3(1), dim3(kClsSlots), 0, s, B);
+    hipLaunchKernelGGL(k_cls_insert, dim3(np), dim3(kBlock), 0, s, G, B);
+    hipLaunchKernelGGL(k_cls_finish, dim3(1), dim3(kClsSlots), 0, s, B, sm, allow_classes);
+    hipLaunchKernelGGL(k_pack_static, dim3(np), dim3(kBlock), 0, s, G, const_cast<float*>(G.pk), N,
+                       sm, B);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_error(SWE_EHIP, std::string("prepare_static_record: ") + hipGetErrorString(e));
+  return SWE_OK;
+}
+
 }  // namespace swe
 
 // ===========================================================================
@@ -781,12 +822,8 @@ int ensure_work(SweContext* c) {
     c->G.pk = c->pk;
   }
   if (!c->cls_mem) {
-    constexpr size_t kKeys = sizeof(unsigned long long) * kClsSlots, kVals = sizeof(float) * 256;
-    SWE_HIP_TRY(hipMalloc(&c->cls_mem, kKeys + kVals + kClsSlots + sizeof(int)));
-    char* m = static_cast<char*>(c->cls_mem);
-    c->cls = ClassBuf{reinterpret_cast<unsigned long long*>(m), reinterpret_cast<float*>(m + kKeys),
-                      reinterpret_cast<unsigned char*>(m + kKeys + kVals),
-                      reinterpret_cast<int*>(m + kKeys + kVals + kClsSlots)};
+    SWE_HIP_TRY(hipMalloc(&c->cls_mem, class_buf_bytes()));
+    c->cls = class_buf_at(c->cls_mem);
     c->G.cls_tab = c->cls.vals;
   }
   return SWE_OK;
@@ -798,29 +835,24 @@ int ensure_work(SweContext* c) {
 // The FP64 sum of the rain multiplier over active cells into d_sum_mult (the
 // fields may have been changed in place since the last run).  Uses the
 // partials as scratch.
+// The partials' vol and clip arrays (reduce_blocks() doubles each) are the
+// scratch of the multiplier sum.
 int refresh_sum_mult(SweContext* c, hipStream_t s) {
-  const int nb = c->reduce_blocks();
-  // The partials' clip array (nb doubles) holds the 2 nb bit-pattern bounds.
-  unsigned* bits = reinterpret_cast<unsigned*>(c->P.clip);
-  hipLaunchKernelGGL(k_sum_mult, dim3(nb), dim3(kBlock), 0, s, c->G, c->P.vol, bits);
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, c->P.vol, bits, nb,
-                     c->d_sum_mult);
-  SWE_HIP_TRY(hipGetLastError());
-  return SWE_OK;
+  return prepare_static_record(c->G, nullptr, c->d_sum_mult, 0, c->P.vol,
+                               reinterpret_cast<unsigned*>(c->P.clip), c->reduce_blocks(), s);
 }
 
 int stage_in(SweContext* c, const SweState& src, hipStream_t s) {
-  if (int rc = refresh_sum_mult(c, s)) return rc;
   c->loop_packed = c->packed_variant();
-  if (!c->loop_packed) return copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), src.h, src.hu, src.hv, s);
+  if (!c->loop_packed) {
+    if (int rc = refresh_sum_mult(c, s)) return rc;
+    return copy_state(c, c->buf(0, 0), c->buf(0, 1), c->buf(0, 2), src.h, src.hu, src.hv, s);
+  }
+  if (int rc = prepare_static_record(c->G, c->cls_mem, c->d_sum_mult, c->static_classes, c->P.vol,
+                                     reinterpret_cast<unsigned*>(c->P.clip), c->reduce_blocks(), s))
+    return rc;
   const size_t N = c->cells();
   const int nb = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 8192);
-  hipLaunchKernelGGL(k_cls_clear, dim3(1), dim3(kClsSlots), 0, s, c->cls);
-  hipLaunchKernelGGL(k_cls_insert, dim3(nb), dim3(kBlock), 0, s, c->G, c->cls);
-  hipLaunchKernelGGL(k_cls_finish, dim3(1), dim3(kClsSlots), 0, s, c->cls, c->d_sum_mult,
-                     c->static_classes);
-  hipLaunchKernelGGL(k_pack_static, dim3(nb), dim3(kBlock), 0, s, c->G, c->pk, N, c->d_sum_mult,
-                     c->cls);
   hipLaunchKernelGGL(k_pack_state, dim3(nb), dim3(kBlock), 0, s, src.h, src.hu, src.hv, c->ws(0), N);
   SWE_HIP_TRY(hipGetLastError());
   return SWE_OK;

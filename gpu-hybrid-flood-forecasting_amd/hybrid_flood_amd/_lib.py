@@ -186,6 +186,7 @@ SIGNATURES = {
     "swe_ensemble_end": (c_int32, [c_void_p, POINTER(SweRunInfo), c_void_p]),
     "swe_ensemble_run": (c_int32, [c_void_p, c_double, c_int32, POINTER(SweRunInfo), c_void_p]),
     "swe_ensemble_total_waves": (c_int32, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
+    "swe_ensemble_get_static_layout": (c_int32, [c_void_p, c_int32, POINTER(c_int32)]),
 }
 
 _lib = None

@@ -84,6 +84,14 @@ class EnsembleRunner:
                    "total_waves")
         return w.value, g.value
 
+    def static_layout(self, member: int) -> int:
+        """The packed static record member ``member``'s last start chose (fast
+        variant): 2 = bed + Manning class, 1 = (bed, n^2), 0 = (bed, n^2, multiplier)."""
+        v = ctypes.c_int32()
+        _lib.check(_lib.load().swe_ensemble_get_static_layout(self.handle, int(member), ctypes.byref(v)),
+                   "swe_ensemble_get_static_layout")
+        return v.value
+
     def _infos(self):
         return (_lib.SweRunInfo * len(self.members))()
 

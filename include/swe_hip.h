@@ -344,6 +344,11 @@ int swe_ensemble_run(SweEnsemble* ens, double t_end, int32_t max_steps, SweRunIn
                      void* stream);
 /* Strips (wavefronts of work) per launch and the launch's workgroup count. */
 int swe_ensemble_total_waves(const SweEnsemble* ens, int32_t* waves, int32_t* grid);
+/* The static record member `member`'s last start chose (fast variant; the
+ * codes of swe_get_static_layout: 2 = bed + Manning class, 1 = (bed, n^2),
+ * 0 = (bed, n^2, multiplier)).  Synchronises the device before it reads the
+ * record.  No reference counterpart. */
+int swe_ensemble_get_static_layout(SweEnsemble* ens, int32_t member, int32_t* layout);
 
 /* ---- Hybrid coupled forecast (hybrid/coupled_forecast.py) ---------------- */
 

@@ -58,6 +58,45 @@ def test_ensemble_members_equal_solo_runs(cuda, variant, rows):
         assert info["volume_final"] == pytest.approx(r.diagnostics["volume_final"], rel=1e-12)
 
 
+def test_ensemble_static_records_equal_solo_runs(cuda):
+    """Members whose fields select each packed static record in one ensemble
+    (bed + Manning class: a two-value roughness map with a uniform multiplier,
+    also on a grid narrower than one 64-column class block; (bed, n^2): many
+    roughness values, uniform multiplier; the full record: a multiplier field):
+    each member reports the record its start chose and equals its solo run
+    through the single-domain loop bit for bit."""
+    import torch
+
+    from hybrid_flood_amd import numerics as nm
+    from hybrid_flood_amd.ensemble import EnsembleRunner
+
+    specs = [((48, 130), "classes", 2), ((70, 5), "classes", 2), ((33, 90), "uniform", 1),
+             ((40, 70), "field", 0), ((1, 77), "classes", 2)]
+    cases = []
+    for k, (shape, kind, _) in enumerate(specs):
+        c = random_case(600 + k, shape, open_sides=(0, 1, 0, 0), speed=0.8)
+        rng = np.random.default_rng(700 + k)
+        if kind == "classes":
+            c["manning_n"] = np.where(rng.uniform(size=shape) < 0.5, 0.015, 0.035).astype(np.float32)
+        if kind in ("classes", "uniform"):
+            c["rainfall_multiplier"] = np.where(c["domain_mask"], 0.7, 0.0).astype(np.float32)
+        cases.append(c)
+    members = [_member(c, cuda) for c in cases]
+    t_end = 2.0
+    solo = [nm.integrate_to_outputs(m.state, m.params, np.array([0.0, t_end], np.float32),
+                                    max_steps=5000, variant="fast") for m in members]
+    ens = EnsembleRunner(members, variant="fast")
+    ens.set_tile_rows(7)
+    infos = ens.run(t_end, max_steps=5000)
+    torch.cuda.synchronize()
+    for k, (m, r, info, (_, kind, lay)) in enumerate(zip(members, solo, infos, specs)):
+        assert ens.static_layout(k) == lay, (k, kind)
+        assert info["steps_used"] == r.steps_used > 3, k
+        for got, want in zip(m.state, r.states):
+            assert torch.equal(got, want[-1]), f"member {k} ({kind}) differs from its solo run"
+        assert info["rain_volume"] == r.diagnostics["rain_volume"], k
+
+
 def test_ensemble_respects_max_steps(cuda):
     from hybrid_flood_amd.ensemble import EnsembleRunner
 
