@@ -51,6 +51,18 @@ __device__ __forceinline__ unsigned long long active_lanes(float np) {
   return __builtin_amdgcn_sicmp(__float_as_int(np), -1, kSignedGreater);
 }
 
+// a[lane + 1] - b (lane 63: 0 - b), as one subtraction with a DPP operand.
+// Written out because the compiler keeps some of these as a move and a
+// subtraction.  A DPP operand must not be read within two wait states of the
+// vector instruction that wrote it (the s_nop).
+__device__ __forceinline__ float east_minus(float a, float b) {
+  float d;
+  asm("s_nop 1\n\tv_sub_f32_dpp %0, %1, %2 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "=v"(d)
+      : "v"(a), "v"(b));
+  return d;
+}
+
 struct SideF {
   float h, eta, b, u, v;
   float p;  // hydrostatic pressure term g/2 h^2, as (g/2 h) h
@@ -112,23 +124,22 @@ __device__ __forceinline__ FaceFlux face_fast(const Grid& G, float hA, float eta
   const float prA = (G.half_g * hrA) * hrA, prB = (G.half_g * hrB) * hrB;
   const float FnA = fmaf(QnA, vnA, prA);
   const float FnB = fmaf(QnB, vnB, prB);
-  float FtA, FtB, dQt;
-  if (XDIR) {
-    const float vtA = wA ? utA : 0.0f, vtB = wB ? utB : 0.0f;
-    FtA = QnA * vtA;
-    FtB = QnB * vtB;
-    dQt = fmaf(hrB, utB, -hrA * utA);
-  } else {
-    const float QtA = hrA * utA, QtB = hrB * utB;
-    FtA = QtA * vnA;
-    FtB = QtB * vnB;
-    dQt = QtB - QtA;
-  }
+  // Tangential flux hr un vt of each side as (hr ut) vn: the same three
+  // factors (vn and vt are zero together, where the side is not wet), one
+  // select fewer than with vt, and the same form for x and y faces.
+  const float QtA = hrA * utA, QtB = hrB * utB;
+  const float FtA = QtA * vnA;
+  const float FtB = QtB * vnB;
+  const float dQt = QtB - QtA;
+  // sR >= 0 >= sL.  Where both wave speeds vanish (spread <= 1e-12: both
+  // sides dry to within (1e-12)^2 / g) the reference takes the quotient as 1;
+  // clamping the spread at 1e-12 leaves such a face's flux as negligible
+  // (every term carries a factor hr <= 1e-25) with one instruction fewer.
   const float spread = sR - sL;
-  const float r = __builtin_amdgcn_rcpf(spread > 1.0e-12f ? spread : 1.0f);
+  const float r = __builtin_amdgcn_rcpf(fmaxf(spread, 1.0e-12f));
   const float ka = sR * r;          // weight of the A-side flux
   const float kb = -sL * r;         // weight of the B-side flux
-  const float kc = (sL * sR) * r;   // weight of the state jump
+  const float kc = ka * sL;         // weight of the state jump, sL sR / (sR - sL)
   FaceFlux F;
   // The mass flux of a wall face (one side mirrored: hrB = hrA, QnB = -QnA,
   // ka = kb) must be exactly zero: two separately rounded products cancel
@@ -149,6 +160,47 @@ __device__ __forceinline__ FaceFlux face_fast(const Grid& G, float hA, float eta
     F.cA = rel ? F.cA : 0.0f;
     F.cB = rel ? F.cB : 0.0f;
   }
+  return F;
+}
+
+// The x face west of this lane's cell when both of its cells are active:
+// side A is lane - 1's cell, side B this lane's.  The same operations as
+// face_fast<true, false> with A = from_west of each value, so the same bits;
+// here each of A's values is fetched at its use, so that the compiler folds
+// the cross-lane move into the instruction that uses it (a DPP operand of a
+// subtraction or product) instead of issuing it as a move of its own.
+__device__ __forceinline__ FaceFlux face_x_west(const Grid& G, const SideF& B) {
+  const float dry = G.dry;
+  const float bf = max_nc(from_west(B.b), B.b);
+  const float hrA = fmaxf(from_west(B.eta) - bf, 0.0f);
+  const float hrB = fmaxf(B.eta - bf, 0.0f);
+  const bool wA = hrA > dry, wB = hrB > dry;
+  const float unA = from_west(B.u);
+  const float vnA = wA ? unA : 0.0f, vnB = wB ? B.u : 0.0f;
+  const float QnA = hrA * unA, QnB = hrB * B.u;
+  const float cA = __builtin_amdgcn_sqrtf(G.g * hrA);
+  const float cB = __builtin_amdgcn_sqrtf(G.g * hrB);
+  const float sL = fminf(fminf(vnA - cA, vnB - cB), 0.0f);
+  const float sR = fmaxf(fmaxf(vnA + cA, vnB + cB), 0.0f);
+  const float prA = (G.half_g * hrA) * hrA, prB = (G.half_g * hrB) * hrB;
+  const float FnA = fmaf(QnA, vnA, prA);
+  const float FnB = fmaf(QnB, vnB, prB);
+  const float QtA = from_west(B.v) * hrA, QtB = hrB * B.v;  // hr ut (x face: ut = v)
+  const float FtA = QtA * vnA;
+  const float FtB = QtB * vnB;
+  const float dQt = QtB - QtA;
+  const float spread = sR - sL;
+  const float r = __builtin_amdgcn_rcpf(fmaxf(spread, 1.0e-12f));
+  const float ka = sR * r;
+  const float kb = -sL * r;
+  const float kc = ka * sL;
+  FaceFlux F;
+  const float mass = fmaf(ka, QnA, kb * QnB);
+  F.f0 = fmaf(kc, hrB - hrA, mass);
+  F.fn = fmaf(kc, QnB - QnA, fmaf(ka, FnA, kb * FnB));
+  F.ft = fmaf(kc, dQt, fmaf(ka, FtA, kb * FtB));
+  F.cA = from_west(B.p) - prA;
+  F.cB = B.p - prB;
   return F;
 }
 
@@ -461,6 +513,11 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   // bound (+inf) is a scalar, and a med3 takes one scalar operand at most.
   float dry_v = dry;
   asm("" : "+v"(dry_v));
+  // With the uniform multiplier (a scalar) the rain term's two factors would
+  // both be scalars, which one vector instruction cannot read: dt * rate is
+  // kept in a vector register for the march, not copied into one per row.
+  float dtrate_v = dtrate;
+  if constexpr (LAY != kStatic12) asm("" : "+v"(dtrate_v));
   using IO = std::conditional_t<PACKED, PackedIO, StripIO>;
   IO io;
   if constexpr (PACKED)
@@ -554,18 +611,16 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     // x face between lane-1 (A) and this lane (B).  Every face of lanes
     // 1..63 joins two active cells when the active mask, shifted by one
     // lane, still covers those lanes: scalar work on the ballot.
-    const float weta = from_west(cur.s.eta), wb = from_west(cur.s.b), wu = from_west(cur.s.u),
-                wv = from_west(cur.s.v), wp = from_west(cur.s.p);
     const unsigned long long am = __builtin_amdgcn_ballot_w64(ca);
     const unsigned long long need = __builtin_amdgcn_read_exec() & ~1ull;
     FaceFlux fx;
     if (((am << 1) & am & need) == need) {
-      fx = face_fast<true, false>(G, 0.0f, weta, wb, wu, wv, wp, true, cur.s.h, cur.s.eta,
-                                  cur.s.b, cur.s.u, cur.s.v, cur.s.p, true);
+      fx = face_x_west(G, cur.s);
     } else {
       const bool wa = from_west((int)ca) != 0;
-      fx = face_fast<true, true>(G, from_west(cur.s.h), weta, wb, wu, wv, wp, wa, cur.s.h,
-                                 cur.s.eta, cur.s.b, cur.s.u, cur.s.v, cur.s.p, ca);
+      fx = face_fast<true, true>(G, from_west(cur.s.h), from_west(cur.s.eta), from_west(cur.s.b),
+                                 from_west(cur.s.u), from_west(cur.s.v), from_west(cur.s.p), wa,
+                                 cur.s.h, cur.s.eta, cur.s.b, cur.s.u, cur.s.v, cur.s.p, ca);
     }
     const float xr0 = fx.f0, xr1 = fx.fn + fx.cB, xr2 = fx.ft;
     const float xe1 = fx.fn + fx.cA;  // this lane's share of the face west of lane+1
@@ -602,16 +657,13 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     const float yn0 = SOUTH ? yc0 : yf0, yn1 = SOUTH ? yc1 : yf1, yn2 = SOUTH ? yc2 : yf2;  // north
     const float ys0 = SOUTH ? yf0 : yc0, ys1 = SOUTH ? yf1 : yc1, ys2 = SOUTH ? yf2 : yc2;  // south
 
-    // The east faces from lane+1, moved next to their one use (the
-    // subtraction below takes the cross-lane move as its operand).
-    const float xl0 = from_east(fx.f0);
-    const float xl1 = from_east(xe1);
-    const float xl2 = from_east(fx.ft);
+    // The east faces from lane+1, each fetched at its one use: a
+    // subtraction that takes the cross-lane move as its (DPP) operand.
     // dt ((xl - xr)/dx + (yn - ys)/dy) as (dt/dx) ((xl - xr) + (yn - ys) dx/dy)
-    const float d0 = fmaf(yn0 - ys0, G.rhox, xl0 - xr0);
-    const float d1 = fmaf(yn1 - ys1, G.rhox, xl1 - xr1);
-    const float d2 = fmaf(yn2 - ys2, G.rhox, xl2 - xr2);
-    const float hs = fmaf(dtrate, mult_j, fmaf(-dtx, d0, cur.s.h));
+    const float d0 = fmaf(yn0 - ys0, G.rhox, east_minus(fx.f0, xr0));
+    const float d1 = fmaf(yn1 - ys1, G.rhox, east_minus(xe1, xr1));
+    const float d2 = fmaf(yn2 - ys2, G.rhox, east_minus(fx.ft, xr2));
+    const float hs = fmaf(dtrate_v, mult_j, fmaf(-dtx, d0, cur.s.h));
     const float hus = fmaf(-dtx, d1, cur.hu);
     const float hvs = fmaf(-dtx, d2, cur.hv);
     const float hn = fmaxf(hs, 0.0f);
