@@ -23,7 +23,9 @@ step's streams without its arithmetic) as the achievable-bandwidth ceiling;
 the batched ensemble (64 uneven urban members, 3600 s each) and the
 checkpointed adjoint (1024^2 urban, dry start with rain, ~2000 steps, with a
 frozen-schedule finite-difference check) against the plain step; the FP64
-mass balance of each benchmark configuration; and, in the ``cpu_baseline``
+mass balance of each benchmark configuration (configurations 2 and 4 also
+with the rain source rounded stochastically, ``swe_set_rain_rounding``); and,
+in the ``cpu_baseline``
 leg, the NumPy FP32 oracle timed on the host (a few steps of the 4096^2
 workload, and the 256^2 sloped-plane configuration in full, median of reps)
 together with the accuracy record of the GPU's run of that configuration
@@ -342,9 +344,10 @@ def accuracy_record(cfg1: dict) -> dict:
     return rec
 
 
-def config2_mass_balance(variant: str) -> dict:
+def config2_mass_balance(variant: str, rain_rounding: str = "nearest") -> dict:
     """The 1024^2 urban configuration (one open edge, 1-min rain series over
-    10800 s, 181 outputs) on the GPU: steps, wall time and FP64 mass balance."""
+    10800 s, 181 outputs) on the GPU: steps, wall time and FP64 mass balance,
+    with the rain rounded to nearest (the reference's) or stochastically."""
     import numpy as np
     import torch
 
@@ -362,10 +365,12 @@ def config2_mass_balance(variant: str) -> dict:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = nm.integrate_to_outputs(nm.SWEState(h, torch.zeros_like(h), torch.zeros_like(h)), params,
-                                times, max_steps=200000, variant=variant)
+                                times, max_steps=200000, variant=variant,
+                                rain_rounding=rain_rounding)
     wall = time.perf_counter() - t0
     d = r.diagnostics
-    return {"outputs_written": int(r.outputs_written), "steps": int(r.steps_used), "wall_s": wall,
+    return {"rain_rounding": rain_rounding, "outputs_written": int(r.outputs_written),
+            "steps": int(r.steps_used), "wall_s": wall,
             "mass_balance_rel": d["mass_balance_rel"], "nonfinite": d["nonfinite"]}
 
 
@@ -386,45 +391,63 @@ def bench_ensemble(variant: str, count: int, seed: int, t_end: float, plain_cups
         members.append(member_from_case(case, device=dev, n_scale=n_scale, rain_scale=rain_scale))
         sides.append(side)
     gen_s = time.perf_counter() - t0
+    start = [m.state.h.clone() for m in members]  # hu = hv = 0 at the start
     runner = EnsembleRunner(members, variant=variant)
     tune = tune_ensemble_rows(runner)
     chunk = 32
     runner.set_graph_chunk(chunk)
     total_cells = int(sum(m.params.bed.numel() for m in members))
-    torch.cuda.synchronize(dev)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    evf = torch.cuda.Event(enable_timing=True)
-    w0 = time.perf_counter()
-    ev0.record()
-    runner.begin(t_end, max_steps=200_000)
-    launches = 0
-    full_launches = 0  # launches while every member was still running
-    last = count
-    full_open = True
-    while True:
-        runner.advance(chunk)
-        launches += chunk
-        running = runner.running()
-        if full_open and running == count:
-            full_launches = launches
-            evf.record()
-        else:
-            full_open = False
-        if running == 0:
-            break
-        if running < last:
-            runner.compact()
-            last = running
-    ev1.record()
-    torch.cuda.synchronize(dev)
-    wall = time.perf_counter() - w0
-    rc, infos = runner.end()
-    ms = ev0.elapsed_time(ev1)
-    ms_full = ev0.elapsed_time(evf) if full_launches else float("nan")
+
+    def timed_run():
+        for m, h0 in zip(members, start):
+            m.state.h.copy_(h0)
+            m.state.hu.zero_()
+            m.state.hv.zero_()
+        torch.cuda.synchronize(dev)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        evf = torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        ev0.record()
+        runner.begin(t_end, max_steps=200_000)
+        launches = 0
+        full_launches = 0  # launches while every member was still running
+        last = count
+        full_open = True
+        while True:
+            runner.advance(chunk)
+            launches += chunk
+            running = runner.running()
+            if full_open and running == count:
+                full_launches = launches
+                evf.record()
+            else:
+                full_open = False
+            if running == 0:
+                break
+            if running < last:
+                runner.compact()
+                last = running
+        ev1.record()
+        torch.cuda.synchronize(dev)
+        wall = time.perf_counter() - w0
+        rc, infos = runner.end()
+        ms = ev0.elapsed_time(ev1)
+        ms_full = ev0.elapsed_time(evf) if full_launches else float("nan")
+        return rc, infos, ms, ms_full, wall, launches, full_launches
+
+    rc, infos, ms, ms_full, wall, launches, full_launches = timed_run()
     cell_steps = sum(m.params.bed.numel() * i["steps_used"] for m, i in zip(members, infos))
     cups = cell_steps / (ms * 1e-3)
     cups_full = total_cells * full_launches / (ms_full * 1e-3) if full_launches else None
+    # the same job with every member's rain rounded stochastically
+    runner.set_rain_rounding("stochastic")
+    rc_s, infos_s, ms_s, _, _, _, _ = timed_run()
+    runner.set_rain_rounding("nearest")
+    cell_steps_s = sum(m.params.bed.numel() * i["steps_used"] for m, i in zip(members, infos_s))
+    stochastic = {"status": rc_s, "ms": ms_s, "cell_updates_per_s": cell_steps_s / (ms_s * 1e-3),
+                  "vs_plain_step": cell_steps_s / (ms_s * 1e-3) / plain_cups,
+                  "max_abs_mass_balance_rel": max(abs(i["mass_balance_rel"]) for i in infos_s)}
     return {
         "workload": f"{count} urban members, sides from {{256,512,1024,2048}} (seed {seed}), "
                     f"dx 2 m, {t_end:.0f} s each, per-member CFL dt, graph-captured launches of one "
@@ -440,6 +463,7 @@ def bench_ensemble(variant: str, count: int, seed: int, t_end: float, plain_cups
         "max_member_steps": int(max(i["steps_used"] for i in infos)),
         "rows": tune["rows"], "tune_ms_per_launch": tune.get("candidates"), "status": rc,
         "max_abs_mass_balance_rel": max(abs(i["mass_balance_rel"]) for i in infos),
+        "stochastic_rain": stochastic,
     }
 
 
@@ -766,11 +790,16 @@ def main(argv=None) -> int:
         mb["config1"] = cfg1["diagnostics"]["mass_balance_rel"]
         c2 = config2_mass_balance(args.variant)
         mb["config2"] = c2["mass_balance_rel"]
+        c2s = config2_mass_balance(args.variant, "stochastic")
+        mb["config2_stochastic_rain"] = c2s["mass_balance_rel"]
         line["config2_run"] = c2
+        line["config2_run_stochastic_rain"] = c2s
         _progress("configurations 1 and 2 done")
     if args.ensemble > 0:
         line["ensemble"] = bench_ensemble(args.variant, args.ensemble, 42, args.ensemble_t, value)
         mb["config4_max_abs_over_members"] = line["ensemble"]["max_abs_mass_balance_rel"]
+        mb["config4_max_abs_over_members_stochastic_rain"] = (
+            line["ensemble"]["stochastic_rain"]["max_abs_mass_balance_rel"])
         _progress("ensemble done")
     if args.adjoint_n > 0:
         line["adjoint"] = bench_adjoint(args.variant, args.adjoint_n, args.adjoint_steps, value)

@@ -205,9 +205,52 @@ __device__ __forceinline__ FaceFlux face_flux(const Grid& G, float hA, float unA
   return F;
 }
 
+// ---------------------------------------------------------------------------
+// Stochastic rounding of the rain source (opt-in, no reference counterpart)
+// ---------------------------------------------------------------------------
+// The rain of one step adds the same increment to every ponded cell, so a
+// round-to-nearest addition makes the same error on every cell of a binade
+// and the errors add up over the grid instead of averaging out: this is the
+// float32 state's drift of the FP64 mass balance.  Stochastic rounding
+// rounds a + r to one of its two float neighbours with the probabilities that
+// make the expected result exact, drawing from a hash of the cell's index and
+// the step's index: deterministic, and the same for every tile, sweep order
+// and launch.  The hash is "lowbias32" (C. Wellons, "Prospecting for hash
+// functions", 2018).
+__host__ __device__ __forceinline__ unsigned sr_hash(unsigned x) {
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Seed of step `step` (0-based index of the step in its run).
+__host__ __device__ __forceinline__ unsigned sr_seed(int step) {
+  return sr_hash((unsigned)step * 0x9E3779B9u + 0x632BE5ABu);
+}
+
+// a + r rounded stochastically with the 32 random bits `bits`.  The exact
+// error e of the nearest sum s (TwoSum, exact in float32 without overflow)
+// decides the direction; s moves one unit in the last place toward a + r
+// when u * gap < |e|, u uniform in [0, 1) on 24 bits: probability |e| / gap.
+__device__ __forceinline__ float sr_add(float a, float r, unsigned bits) {
+  const float s = a + r;
+  const float bb = s - a;
+  const float e = (a - (s - bb)) + (r - bb);
+  // away from zero when e has the sign of s, else toward it (s = 0 gives e = 0)
+  const int dir = (e > 0.0f) == (s >= 0.0f) ? 1 : -1;
+  const float t = __int_as_float(__float_as_int(s) + dir);
+  const float gap = fabsf(t - s);
+  const float u = (float)(bits >> 8) * 5.9604644775390625e-8f;  // 2^-24
+  return (e != 0.0f && u * gap < fabsf(e)) ? t : s;
+}
+
 // Pointwise part of the update (numerics.py:350-369) for one cell, given the
 // flux divergences.  Returns the new state; h_star (before the clip) is kept
-// for the mass balance.
+// for the mass balance.  sr: the rain is added with sr_add and the random
+// bits `srb` (the reference adds it rounded to nearest).
 struct CellUpdate {
   float h, hu, hv, h_star;
   bool wet;
@@ -217,12 +260,13 @@ template <class M>
 __device__ __forceinline__ CellUpdate cell_update(const Grid& G, float h, float hu, float hv,
                                                   float div0, float div1, float div2, float n,
                                                   float mult, int active, float dt,
-                                                  float dtrate, float dtg) {
+                                                  float dtrate, float dtg, bool sr = false,
+                                                  unsigned srb = 0u) {
   CellUpdate r;
   float hs = h - dt * div0;
   const float hus = hu - dt * div1;
   const float hvs = hv - dt * div2;
-  hs = hs + dtrate * mult;
+  hs = sr ? sr_add(hs, dtrate * mult, srb) : hs + dtrate * mult;
   const float hn = fmaxf(hs, 0.0f);
   const float q = M::sqrt(hus * hus + hvs * hvs);
   const float num = (dtg * (n * n)) * q;

@@ -116,6 +116,8 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
     const int par = ctl->steps & 1;
     const float* src = mb.buf[par];
     float* dst = mb.buf[par ^ 1];
+    // the member's rain added with stochastic rounding (drawn for its step index)
+    const int sr_step = (mb.L.rain_sr && ctl->dtrate_f != 0.0f) ? ctl->steps : -1;
     StripAcc acc;
     if constexpr (M::kFast && PACKED) {
       // The member's static record, chosen at its start (swe_ensemble_begin):
@@ -137,27 +139,29 @@ __global__ __launch_bounds__(256) void k_ensemble_step(const Member* __restrict_
         march_strip_fast<true, true, kStaticCls>(mb.G, src, nullptr, nullptr, dst, nullptr,
                                                  nullptr, sx, sy, mb.rows, ctl->dtx_f,
                                                  ctl->dtrate_f, dtg, mb.L.dxd, mb.L.dyd, 0, 0,
-                                                 nullptr, nullptr, nullptr, acc, mc, tab);
+                                                 nullptr, nullptr, nullptr, acc, mc, tab, sr_step);
       } else if (lay == kStatic8) {
         march_strip_fast<true, true, kStatic8>(mb.G, src, nullptr, nullptr, dst, nullptr, nullptr,
                                                sx, sy, mb.rows, ctl->dtx_f, ctl->dtrate_f, dtg,
                                                mb.L.dxd, mb.L.dyd, 0, 0, nullptr, nullptr, nullptr,
-                                               acc, mc);
+                                               acc, mc, nullptr, sr_step);
       } else {
         march_strip_fast<true, true, kStatic12>(mb.G, src, nullptr, nullptr, dst, nullptr,
                                                 nullptr, sx, sy, mb.rows, ctl->dtx_f,
                                                 ctl->dtrate_f, dtg, mb.L.dxd, mb.L.dyd, 0, 0,
-                                                nullptr, nullptr, nullptr, acc);
+                                                nullptr, nullptr, nullptr, acc, 0.0f, nullptr,
+                                                sr_step);
       }
     } else if constexpr (M::kFast)
       march_strip_fast<true, false>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
                                     local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dtx_f,
                                     ctl->dtrate_f, ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr,
-                                    nullptr, nullptr, acc);
+                                    nullptr, nullptr, acc, 0.0f, nullptr, sr_step);
     else
       march_strip<M, true>(mb.G, src, src + N, src + 2 * N, dst, dst + N, dst + 2 * N,
                            local % mb.segs_x, local / mb.segs_x, mb.rows, ctl->dt_f, ctl->dtrate_f,
-                           ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr, nullptr, nullptr, acc);
+                           ctl->dtg_f, mb.L.dxd, mb.L.dyd, 0, 0, nullptr, nullptr, nullptr, acc,
+                           sr_step);
     // Plain stores: the finish kernel reads them after the kernel boundary.
     const float inv = wave_max(acc.inv);
     const double vol = wave_sum(acc.vol);
@@ -537,6 +541,7 @@ int swe_ensemble_create(const SweMemberDesc* members, int32_t n, int32_t variant
     L.n_rain = p.n_rainfall;
     L.n_times = 2;
     L.record = 0;
+    L.rain_sr = 0;
     for (int k = 0; k < p.n_rainfall; ++k) rain_all.push_back(p.rainfall_times_s[k]);
     for (int k = 0; k < p.n_rainfall; ++k) rain_all.push_back(p.rainfall_rates_m_s[k]);
     rain_off += 2 * (size_t)p.n_rainfall;
@@ -585,6 +590,14 @@ int swe_ensemble_set_tile_rows(SweEnsemble* e, int32_t rows) {
   if (e->active) return efail(SWE_EINVAL, "swe_ensemble_set_tile_rows: not while a run is active");
   e->rows = rows;
   return layout(e);
+}
+
+int swe_ensemble_set_rain_rounding(SweEnsemble* e, int32_t mode) {
+  if (!e || (mode != 0 && mode != 1))
+    return efail(SWE_EINVAL, "swe_ensemble_set_rain_rounding: mode must be 0 (nearest) or 1 (stochastic)");
+  if (e->active) return efail(SWE_EINVAL, "swe_ensemble_set_rain_rounding: not while a run is active");
+  for (Member& mb : e->host) mb.L.rain_sr = mode;
+  return layout(e);  // the device member table carries the flag
 }
 
 int swe_ensemble_set_graph_chunk(SweEnsemble* e, int32_t chunk) {

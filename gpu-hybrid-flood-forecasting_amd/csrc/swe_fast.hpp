@@ -486,7 +486,7 @@ __device__ __forceinline__ void static_for(F&& f) {
 // row j+2 are in flight while row j is computed.  PACKED: H and Ho are packed
 // states (3 floats per cell, HU/HV unused) and the static fields come from
 // G.pk; the arithmetic is the same, so both layouts give identical bits.
-template <bool LOOP, bool PACKED, bool SOUTH, int LAY, bool EDGE, bool VOL>
+template <bool LOOP, bool PACKED, bool SOUTH, int LAY, bool EDGE, bool VOL, bool SR>
 __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __restrict__ H,
                                                  const float* __restrict__ HU,
                                                  const float* __restrict__ HV,
@@ -497,7 +497,7 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
                                                  float* __restrict__ cube_h,
                                                  float* __restrict__ cube_hu,
                                                  float* __restrict__ cube_hv, StripAcc& acc,
-                                                 float mc, const float* tab) {
+                                                 float mc, const float* tab, int sr_step) {
   const size_t N = (size_t)G.nx * (size_t)G.ny;
   const int lane = threadIdx.x & (kWave - 1);
   const int i = sx * kStripCols - 1 + lane;
@@ -518,6 +518,9 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
   // kept in a vector register for the march, not copied into one per row.
   float dtrate_v = dtrate;
   if constexpr (LAY != kStatic12) asm("" : "+v"(dtrate_v));
+  // SR: the rain is added with stochastic rounding (sr_add), drawn for
+  // (cell, step sr_step).
+  const unsigned sr_seed_s = SR ? __builtin_amdgcn_readfirstlane(sr_seed(sr_step)) : 0u;
   using IO = std::conditional_t<PACKED, PackedIO, StripIO>;
   IO io;
   if constexpr (PACKED)
@@ -663,10 +666,19 @@ __device__ __forceinline__ void march_strip_dir(const Grid& G, const float* __re
     const float d0 = fmaf(yn0 - ys0, G.rhox, east_minus(fx.f0, xr0));
     const float d1 = fmaf(yn1 - ys1, G.rhox, east_minus(xe1, xr1));
     const float d2 = fmaf(yn2 - ys2, G.rhox, east_minus(fx.ft, xr2));
-    const float hs = fmaf(dtrate_v, mult_j, fmaf(-dtx, d0, cur.s.h));
+    const float hpre = fmaf(-dtx, d0, cur.s.h);
+    float hs;
+    if constexpr (SR) {
+      const unsigned cell = (unsigned)j * (unsigned)G.nx + (unsigned)ic;
+      hs = sr_add(hpre, dtrate_v * mult_j, sr_hash(cell ^ sr_seed_s));
+    } else {
+      hs = fmaf(dtrate_v, mult_j, hpre);
+    }
     const float hus = fmaf(-dtx, d1, cur.hu);
     const float hvs = fmaf(-dtx, d2, cur.hv);
-    const float hn = fmaxf(hs, 0.0f);
+    // (sr_add's result is a selection of bit patterns, which the compiler
+    // would quiet before a plain max: max_nc takes it as it is)
+    const float hn = SR ? max_nc(hs, 0.0f) : fmaxf(hs, 0.0f);
     const float hd = max_nc(hs, dry_v);  // = max(hn, dry): dry > 0
     const float rhd = __builtin_amdgcn_rcpf(hd);
     const float q = __builtin_amdgcn_sqrtf(fmaf(hus, hus, hvs * hvs));
@@ -800,15 +812,31 @@ __device__ __forceinline__ void march_strip_fast(const Grid& G, const float* __r
                                                  float* __restrict__ cube_h,
                                                  float* __restrict__ cube_hu,
                                                  float* __restrict__ cube_hv, StripAcc& acc,
-                                                 float mc = 0.0f, const float* tab = nullptr) {
+                                                 float mc = 0.0f, const float* tab = nullptr,
+                                                 int sr_step = -1) {
   static_assert(PACKED || LAY == kStatic12, "the compact static records are packed layouts");
-  auto march = [&](auto south, auto edge) __attribute__((always_inline)) {
-    march_strip_dir<LOOP, PACKED, decltype(south)::value, LAY, decltype(edge)::value, VOL>(
-        G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate, dtg, dxd, dyd, rec_lo, rec_hi,
-        cube_h, cube_hu, cube_hv, acc, mc, tab);
-  };
   using S = std::true_type;
   using N = std::false_type;
+  // A device loop's step with stochastic rounding of its rain (sr_step >= 0)
+  // takes its own instantiation of the march, so the other steps' code does
+  // not change.
+  auto march = [&](auto south, auto edge) __attribute__((always_inline)) {
+    if constexpr (LOOP) {
+      if (sr_step >= 0)
+        march_strip_dir<LOOP, PACKED, decltype(south)::value, LAY, decltype(edge)::value, VOL,
+                        true>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate, dtg, dxd,
+                              dyd, rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc, mc, tab,
+                              sr_step);
+      else
+        march_strip_dir<LOOP, PACKED, decltype(south)::value, LAY, decltype(edge)::value, VOL,
+                        false>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate, dtg, dxd,
+                               dyd, rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc, mc, tab, -1);
+    } else {
+      march_strip_dir<LOOP, PACKED, decltype(south)::value, LAY, decltype(edge)::value, VOL,
+                      false>(G, H, HU, HV, Ho, HUo, HVo, sx, sy, rows, dtx, dtrate, dtg, dxd, dyd,
+                             rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc, mc, tab, -1);
+    }
+  };
   if constexpr (PACKED) {
     // A strip whose 64 columns and two halo rows all lie inside the grid
     // (every strip but the first and last of a row band, in every band but

@@ -80,6 +80,7 @@ struct LoopConst {
   int n_rain, n_times, max_steps, dt_log_cap;
   int ck_every, ck_cap;
   int record;      // 0: output times only stop/clamp the loop, no state is stored
+  int rain_sr;     // 1: a step with rain adds it with stochastic rounding (sr_add)
 };
 
 struct Partials {

@@ -53,6 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
   float dt = dt_arg, dtx = dtx_arg, dtrate = dtrate_arg, dtg = dtg_arg;
   int rec_lo = 0, rec_hi = 0;
   int steps_before = 0;  // fused loop: index of this step
+  int sr_step = -1;      // device loop, rain rounded stochastically: index of this step
   int lay = kStatic12;   // packed loop: static record (L.sum_mult[3], set at the loop start)
   float cls_n2 = 0.0f;   // kStaticCls: n^2 of class threadIdx.x (negative: inactive)
   if constexpr (PACKED) {
@@ -107,6 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
     dtg = go.dtg_f;
     rec_lo = go.rec_lo;
     rec_hi = go.rec_hi;
+    if (L.rain_sr && dtrate != 0.0f) sr_step = steps_before;
     if (go.ck_slot >= 0) {
       cube_h = L.ck_h;
       cube_hu = L.ck_hu;
@@ -122,6 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
     dtg = ctl->dtg_f;
     rec_lo = ctl->rec_lo;
     rec_hi = ctl->rec_hi;
+    if (L.rain_sr && dtrate != 0.0f) sr_step = ctl->steps;
     // A checkpoint step (never also an output step) records into the
     // checkpoint slots through the same stores.
     const int ck = ctl->ck_slot;
@@ -158,23 +161,26 @@ __global__ __launch_bounds__(kBlock) void k_step(
         march_strip_fast<true, true, kStaticCls, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
                                                  gw / segs_x, rows, dtx, dtrate, dtg, L.dxd,
                                                  L.dyd, rec_lo, rec_hi, cube_h, cube_hu, cube_hv,
-                                                 acc, mc, class_table());
+                                                 acc, mc, class_table(), sr_step);
       else if (lay == kStatic8)
         march_strip_fast<true, true, kStatic8, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
                                                gw / segs_x, rows, dtx, dtrate, dtg, L.dxd, L.dyd,
-                                               rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc, mc);
+                                               rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc, mc,
+                                               nullptr, sr_step);
       else
         march_strip_fast<true, true, kStatic12, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
                                                 gw / segs_x, rows, dtx, dtrate, dtg, L.dxd, L.dyd,
-                                                rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
+                                                rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc, 0.0f,
+                                                nullptr, sr_step);
     } else if constexpr (M::kFast)
       march_strip_fast<MODE != MODE_STEP, false>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x,
                                                  gw / segs_x, rows, dtx, dtrate, dtg, L.dxd, L.dyd,
-                                                 rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc);
+                                                 rec_lo, rec_hi, cube_h, cube_hu, cube_hv, acc,
+                                                 0.0f, nullptr, sr_step);
     else
       march_strip<M, MODE != MODE_STEP>(G, H, HU, HV, Ho, HUo, HVo, gw % segs_x, gw / segs_x,
                                         rows, dt, dtrate, dtg, L.dxd, L.dyd, rec_lo, rec_hi,
-                                        cube_h, cube_hu, cube_hv, acc);
+                                        cube_h, cube_hu, cube_hv, acc, sr_step);
   }
 
   if constexpr (MODE == MODE_LOOP_SPLIT) {
@@ -713,6 +719,7 @@ struct SweContext {
   // one-workgroup finish kernel after each step, 0 = the step's last workgroup.
   int split_finish = 1;
   int alt_sweep = 1;         // odd device-loop steps sweep the strips in reverse order
+  int rain_sr = 0;           // device loops add the rain with stochastic rounding
   int sweep_rev = 0;         // reverse flag of the step launch being issued
   int step_par = 0;          // parity of the step launch being issued (fused loop)
   float step_dtx = 0.0f;     // float32(dt/dx) of a caller-given step (swe_step, replays)
@@ -768,6 +775,7 @@ struct SweContext {
     L.n_times = n_times;
     L.max_steps = max_steps;
     L.record = 1;
+    L.rain_sr = rain_sr;
     return L;
   }
 };
@@ -1373,6 +1381,16 @@ int swe_get_static_layout(SweContext* c, int32_t* layout) {
   return SWE_OK;
 }
 
+int swe_set_rain_rounding(SweContext* c, int32_t mode) {
+  if (!c || (mode != 0 && mode != 1))
+    return fail(SWE_EINVAL, "swe_set_rain_rounding: mode must be 0 (nearest) or 1 (stochastic)");
+  if (mode == 1 && c->ck_every > 0)
+    return fail(SWE_EINVAL, "swe_set_rain_rounding: not while checkpoints are set");
+  if (c->loop_active) return fail(SWE_EINVAL, "swe_set_rain_rounding: not while a loop is active");
+  c->rain_sr = mode;
+  return SWE_OK;
+}
+
 int swe_set_sweep_order(SweContext* c, int32_t alternate) {
   if (!c || (alternate != 0 && alternate != 1))
     return fail(SWE_EINVAL, "swe_set_sweep_order: alternate must be 0 or 1");
@@ -1385,6 +1403,10 @@ int swe_set_checkpoints(SweContext* c, float* h, float* hu, float* hv, int32_t e
   if (!c || every < 0 || capacity < 0 || ((every > 0 && capacity > 0) && (!h || !hu || !hv)))
     return fail(SWE_EINVAL, "swe_set_checkpoints: bad arguments");
   const bool on = every > 0 && capacity > 0;
+  if (on && c->rain_sr)
+    return fail(SWE_EINVAL,
+                "swe_set_checkpoints: the replays of an adjoint round the rain to nearest; "
+                "set swe_set_rain_rounding(ctx, 0) first");
   c->ck_h = on ? h : nullptr;
   c->ck_hu = on ? hu : nullptr;
   c->ck_hv = on ? hv : nullptr;

@@ -60,9 +60,14 @@ __device__ __forceinline__ void march_strip(const Grid& G, const float* __restri
                                             float dtg, double dxd, double dyd, int rec_lo,
                                             int rec_hi, float* __restrict__ cube_h,
                                             float* __restrict__ cube_hu,
-                                            float* __restrict__ cube_hv, StripAcc& acc) {
+                                            float* __restrict__ cube_hv, StripAcc& acc,
+                                            int sr_step = -1) {
   const size_t N = (size_t)G.nx * (size_t)G.ny;
   const int lane = threadIdx.x & (kWave - 1);
+  // sr_step >= 0: the rain is added with stochastic rounding, drawn for
+  // (cell, step) -- the same draw as the fast march and the oracle.
+  const bool sr = sr_step >= 0;
+  const unsigned seed = sr ? sr_seed(sr_step) : 0u;
   const int i = sx * kStripCols - 1 + lane;
   const bool own = lane >= 1 && lane <= kStripCols && i < G.nx;
   ColumnReader<M> rd;
@@ -131,8 +136,8 @@ __device__ __forceinline__ void march_strip(const Grid& G, const float* __restri
     const float d0 = M::per_dx(xl0 - xr0, G.dx, G.rdx) + M::per_dx(yb0 - yt0, G.dy, G.rdy);
     const float d1 = M::per_dx(xl1 - xr1, G.dx, G.rdx) + M::per_dx(yb1 - yt1, G.dy, G.rdy);
     const float d2 = M::per_dx(xl2 - xr2, G.dx, G.rdx) + M::per_dx(yb2 - yt2, G.dy, G.rdy);
-    const CellUpdate r =
-        cell_update<M>(G, cur.h, cur.hu, cur.hv, d0, d1, d2, nn, mult, cur.a, dt, dtrate, dtg);
+    const CellUpdate r = cell_update<M>(G, cur.h, cur.hu, cur.hv, d0, d1, d2, nn, mult, cur.a,
+                                        dt, dtrate, dtg, sr, sr_hash((unsigned)idx ^ seed));
     if (own) {
       Ho[idx] = r.h;
       HUo[idx] = r.hu;

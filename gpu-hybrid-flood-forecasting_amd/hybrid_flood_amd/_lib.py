@@ -101,6 +101,8 @@ SIGNATURES = {
     "swe_set_loop_finish": (c_int32, [c_void_p, c_int32]),
     "swe_set_static_classes": (c_int32, [c_void_p, c_int32]),
     "swe_get_static_layout": (c_int32, [c_void_p, POINTER(c_int32)]),
+    "swe_set_rain_rounding": (c_int32, [c_void_p, c_int32]),
+    "swe_ensemble_set_rain_rounding": (c_int32, [c_void_p, c_int32]),
     "swe_set_sweep_order": (c_int32, [c_void_p, c_int32]),
     "swe_adjoint_step": (c_int32, [c_void_p, SweState, SweState, SweState, c_double, c_double,
                                    c_void_p, c_void_p, c_void_p]),
@@ -224,6 +226,15 @@ def last_error() -> str:
     return msg.decode() if msg else ""
 
 
+RAIN_ROUNDING = {"nearest": 0, "stochastic": 1}
+
+
+def rain_rounding_code(mode: str) -> int:
+    if mode not in RAIN_ROUNDING:
+        raise ValueError("rain_rounding must be 'nearest' or 'stochastic'")
+    return RAIN_ROUNDING[mode]
+
+
 def check(code: int, where: str) -> int:
     if code == SWE_OK:
         return code
@@ -279,6 +290,7 @@ class Context:
         self.handle = handle
         self.shape = (ny, nx)
         self.variant = variant
+        self.rain_rounding = "nearest"
         self.gravity = float(gravity)
 
     def __del__(self):
@@ -312,6 +324,13 @@ class Context:
         v = c_int32(-1)
         check(load().swe_get_static_layout(self.handle, byref(v)), "swe_get_static_layout")
         return int(v.value)
+
+    def set_rain_rounding(self, mode: str) -> None:
+        """'nearest' (the reference's) or 'stochastic' rounding of the rain
+        source in the device loops (see swe_set_rain_rounding)."""
+        check(load().swe_set_rain_rounding(self.handle, rain_rounding_code(mode)),
+              "swe_set_rain_rounding")
+        self.rain_rounding = mode
 
     def set_sweep_order(self, alternate: int) -> None:
         check(load().swe_set_sweep_order(self.handle, int(alternate)), "swe_set_sweep_order")

@@ -84,6 +84,12 @@ class EnsembleRunner:
                    "total_waves")
         return w.value, g.value
 
+    def set_rain_rounding(self, mode: str) -> None:
+        """'nearest' (the reference's) or 'stochastic' rounding of every
+        member's rain source (swe_ensemble_set_rain_rounding)."""
+        _lib.check(_lib.load().swe_ensemble_set_rain_rounding(self.handle, _lib.rain_rounding_code(mode)),
+                   "swe_ensemble_set_rain_rounding")
+
     def static_layout(self, member: int) -> int:
         """The packed static record member ``member``'s last start chose (fast
         variant): 2 = bed + Manning class, 1 = (bed, n^2), 0 = (bed, n^2, multiplier)."""

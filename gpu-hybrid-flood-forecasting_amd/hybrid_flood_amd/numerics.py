@@ -225,18 +225,30 @@ def _output_times_f64(output_times_s) -> np.ndarray:
 
 
 def integrate_to_outputs(initial_state: SWEState, params: SWEParams, output_times_s, *,
-                         max_steps: int, variant: str | None = None) -> ScanResult:
+                         max_steps: int, variant: str | None = None,
+                         rain_rounding: str | None = None) -> ScanResult:
     """Integrate adaptively to fixed output times (numerics.py:372-434).
 
     The whole loop runs on the device; the host launches chunks of steps and
     polls the stop flag.  Like the reference's scan it does not raise when
     ``max_steps`` ends the loop early: ``outputs_written`` tells.
+    ``rain_rounding`` ("nearest", the reference's, or "stochastic"; None keeps
+    the context's setting) selects how the rain source is rounded for this
+    run (``swe_set_rain_rounding``).
     """
     times = _output_times_f64(output_times_s)
     if times.ndim != 1 or times.shape[0] < 1:
         raise ValueError("At least one one-dimensional output time is required.")
     ctx = context_for(params, variant)
     _check_state(initial_state, ctx.shape)
+    if rain_rounding is not None and rain_rounding != ctx.rain_rounding:
+        previous = ctx.rain_rounding
+        ctx.set_rain_rounding(rain_rounding)
+        try:
+            return integrate_to_outputs(initial_state, params, output_times_s,
+                                        max_steps=max_steps, variant=variant)
+        finally:
+            ctx.set_rain_rounding(previous)
     ny, nx = ctx.shape
     T = times.shape[0]
     dev = initial_state.h.device

@@ -159,6 +159,19 @@ int swe_get_static_layout(SweContext* ctx, int32_t* layout);
  * sweeps in the same order.  States are bit-identical either way.  No
  * reference counterpart (XLA schedules the reference's step). */
 int swe_set_sweep_order(SweContext* ctx, int32_t alternate);
+/* Rounding of the rain source in the device loops (swe_run_to_outputs,
+ * swe_advance_to_time, swe_run_steps, swe_loop_*): 0 (default) = round to
+ * nearest, as the reference; 1 = stochastic: a step with rain adds it to each
+ * cell's depth rounded up or down to a float neighbour with the probabilities
+ * that make the expected depth exact, drawn from a hash of (cell index, step
+ * index) -- deterministic and independent of the tile.  Round-to-nearest adds
+ * the same error on every ponded cell of a binade, which is the float32
+ * state's drift of the FP64 mass balance on long rainy runs; stochastic
+ * rounding decorrelates those errors.  swe_step and the replays of the
+ * adjoint always round to nearest, so it cannot be combined with
+ * swe_set_checkpoints (SWE_EINVAL either way round).  No reference
+ * counterpart. */
+int swe_set_rain_rounding(SweContext* ctx, int32_t mode);
 
 /* One step with caller-given dt and time (finite_volume_step,
  * jax_solver/numerics.py:312-369).  `in` and `out` must not alias. */
@@ -349,6 +362,10 @@ int swe_ensemble_total_waves(const SweEnsemble* ens, int32_t* waves, int32_t* gr
  * 0 = (bed, n^2, multiplier)).  Synchronises the device before it reads the
  * record.  No reference counterpart. */
 int swe_ensemble_get_static_layout(SweEnsemble* ens, int32_t member, int32_t* layout);
+/* Rounding of every member's rain source (swe_set_rain_rounding), drawn for
+ * each member's own step index.  Not while a run is active.  No reference
+ * counterpart. */
+int swe_ensemble_set_rain_rounding(SweEnsemble* ens, int32_t mode);
 
 /* ---- Hybrid coupled forecast (hybrid/coupled_forecast.py) ---------------- */
 

@@ -236,8 +236,56 @@ class StepDiagnostics(NamedTuple):
     rain_volume: float      # rain volume applied to active cells this step (m^3)
 
 
-def finite_volume_step(state, params, dt, time_s, *, norm="sqrt", diagnostics=False):
-    """One conservative, well-balanced HLL step (numerics.py:312-369)."""
+# ----------------------------------------------------------------------------
+# Stochastic rounding of the rain source (an option of this build; the
+# reference rounds to nearest).  Restates the device's sr_hash / sr_seed /
+# sr_add bit for bit: "lowbias32" (C. Wellons, "Prospecting for hash
+# functions", 2018) on uint32, TwoSum for the exact error of the nearest sum,
+# and the float neighbour in the error's direction taken when u * gap < |e|.
+# ----------------------------------------------------------------------------
+_M32 = np.uint64(0xFFFFFFFF)
+
+
+def sr_hash(x):
+    """lowbias32 of uint32 values (array or scalar), as uint32."""
+    x = np.asarray(x, dtype=np.uint64) & _M32
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x7FEB352D)) & _M32
+    x ^= x >> np.uint64(15)
+    x = (x * np.uint64(0x846CA68B)) & _M32
+    x ^= x >> np.uint64(16)
+    return x.astype(np.uint32)
+
+
+def sr_seed(step: int) -> np.uint32:
+    return sr_hash((np.uint64(step) * np.uint64(0x9E3779B9) + np.uint64(0x632BE5AB)) & _M32)
+
+
+def sr_add(a, r, bits):
+    """float32 a + r rounded stochastically with the uint32 random ``bits``."""
+    a = np.asarray(a, np.float32)
+    r = np.asarray(r, np.float32)
+    s = (a + r).astype(np.float32)
+    bb = (s - a).astype(np.float32)
+    e = ((a - (s - bb)).astype(np.float32) + (r - bb).astype(np.float32)).astype(np.float32)
+    d = np.where((e > 0) == (s >= 0), 1, -1).astype(np.int64)
+    t = ((s.view(np.int32).astype(np.int64) + d) & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+    with np.errstate(invalid="ignore"):
+        gap = np.abs((t - s).astype(np.float32))
+        u = ((np.asarray(bits, np.uint32) >> np.uint32(8)).astype(np.float32)
+             * np.float32(2.0 ** -24))
+        take = (e != 0) & ((u * gap).astype(np.float32) < np.abs(e))
+    return np.where(take, t, s).astype(np.float32)
+
+
+def finite_volume_step(state, params, dt, time_s, *, norm="sqrt", diagnostics=False,
+                       sr_step=None):
+    """One conservative, well-balanced HLL step (numerics.py:312-369).
+
+    ``sr_step`` (this build's option): add the rain with :func:`sr_add`, drawn
+    for (cell index, step ``sr_step``), as the device loops do with stochastic
+    rain rounding.
+    """
     T = params.dtype
     g = T(params.gravity)
     half_g = T(0.5 * params.gravity)
@@ -278,7 +326,12 @@ def finite_volume_step(state, params, dt, time_s, *, norm="sqrt", diagnostics=Fa
         upd.append(u0 - dt_a * (div_x + div_y))
     rate = rainfall_rate(params, time_s)
     dtrate = T(dt * rate)
-    h_star = upd[0] + dtrate * params.rainfall_multiplier
+    if sr_step is None:
+        h_star = upd[0] + dtrate * params.rainfall_multiplier
+    else:
+        cells = np.arange(ny * nx, dtype=np.uint64).reshape(ny, nx)
+        h_star = sr_add(upd[0], dtrate * params.rainfall_multiplier,
+                        sr_hash(cells ^ np.uint64(sr_seed(sr_step))))
     h_new = np.maximum(h_star, zero)
     hu_s, hv_s = upd[1], upd[2]
     if norm == "hypot":
@@ -347,7 +400,7 @@ def volume(state: OracleState, params: OracleParams) -> float:
 
 
 def integrate_to_outputs(initial, params, output_times_s, *, max_steps, norm="sqrt",
-                         dt_schedule=None):
+                         dt_schedule=None, rain_rounding="nearest"):
     """Adaptive loop to fixed output times (numerics.py:372-434), float64 clock.
 
     ``dt_schedule`` replays a recorded list of step sizes instead of the CFL
@@ -356,6 +409,10 @@ def integrate_to_outputs(initial, params, output_times_s, *, max_steps, norm="sq
     times = np.asarray(output_times_s, np.float64)
     if times.ndim != 1 or times.shape[0] < 1:
         raise ValueError("At least one one-dimensional output time is required.")
+    if rain_rounding not in ("nearest", "stochastic"):
+        raise ValueError("rain_rounding must be 'nearest' or 'stochastic'")
+    if rain_rounding == "stochastic" and params.dtype != np.float32:
+        raise ValueError("stochastic rain rounding is a float32 option")
     T = params.dtype
     state = OracleState(*(np.asarray(x, T) for x in initial))
     outs = [state]
@@ -371,7 +428,9 @@ def integrate_to_outputs(initial, params, output_times_s, *, max_steps, norm="sq
                 dt = float(dt_schedule[steps])
             else:
                 dt = min(compute_cfl_dt(state, params), float(target) - t)
-            state, diag = finite_volume_step(state, params, dt, t, norm=norm, diagnostics=True)
+            state, diag = finite_volume_step(
+                state, params, dt, t, norm=norm, diagnostics=True,
+                sr_step=steps if rain_rounding == "stochastic" else None)
             rain += diag.rain_volume
             bnd += diag.boundary_inflow
             clip += diag.clip_volume
