@@ -17,7 +17,10 @@ from ctypes import POINTER, byref, c_char_p, c_double, c_float, c_int32, c_int64
 import torch  # noqa: F401  (load torch's HIP runtime before the library)
 
 LIB_NAME = "libswe_hip.so"
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+# SWE_LIB_PATH names another build of the same library (the host-side
+# AddressSanitizer build of `make asan`, for the CPU tests of the C ABI).
+LIB_PATH = os.environ.get("SWE_LIB_PATH") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 SWE_OK = 0
 SWE_EINVAL = 1
