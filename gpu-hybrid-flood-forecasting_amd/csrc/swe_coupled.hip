@@ -450,6 +450,13 @@ int grid_blocks(size_t N) {
   return (int)std::max<size_t>(1, std::min<size_t>((N + kBlock - 1) / kBlock, 4096));
 }
 
+// Elementwise passes: one item per thread (their grid-stride loops then run
+// once), so every load of the pass is in flight at once; a grid capped at a
+// few thousand workgroups made each thread walk several items in turn.
+int stream_blocks(size_t items) {
+  return (int)std::max<size_t>(1, std::min<size_t>((items + kBlock - 1) / kBlock, 0x7FFFFFFF));
+}
+
 bool valid(const SweState& s) { return s.h && s.hu && s.hv; }
 
 // apply_residual_correction on `scratch` (device: 4 result doubles, then 2 x nb
@@ -477,15 +484,15 @@ int correct_with(const Grid& G, SweState predicted, const float* residual, int64
                    al(predicted.h, 16) && al(predicted.hu, 16) && al(predicted.hv, 16) &&
                    al(corrected.h, 16) && al(corrected.hu, 16) && al(corrected.hv, 16);
   if (vec && interleaved)
-    hipLaunchKernelGGL(k_corr_apply4<true>, dim3(grid_blocks(N / 4)), dim3(kBlock), 0, s, G,
+    hipLaunchKernelGGL(k_corr_apply4<true>, dim3(stream_blocks(N / 4)), dim3(kBlock), 0, s, G,
                        predicted.h, predicted.hu, predicted.hv, R, scratch, corrected.h,
                        corrected.hu, corrected.hv);
   else if (vec)
-    hipLaunchKernelGGL(k_corr_apply4<false>, dim3(grid_blocks(N / 4)), dim3(kBlock), 0, s, G,
+    hipLaunchKernelGGL(k_corr_apply4<false>, dim3(stream_blocks(N / 4)), dim3(kBlock), 0, s, G,
                        predicted.h, predicted.hu, predicted.hv, R, scratch, corrected.h,
                        corrected.hu, corrected.hv);
   else
-    hipLaunchKernelGGL(k_corr_apply, dim3(grid_blocks(N)), dim3(kBlock), 0, s, G, predicted.h,
+    hipLaunchKernelGGL(k_corr_apply, dim3(stream_blocks(N)), dim3(kBlock), 0, s, G, predicted.h,
                        predicted.hu, predicted.hv, R, scratch, corrected.h, corrected.hu,
                        corrected.hv);
   hipError_t e = hipGetLastError();
@@ -565,11 +572,11 @@ int swe_network_features(SweContext* c, SweState state, const float* terrain_std
   if (channel_stride == 1 && cell_stride == 6 && N % 4 == 0 && al(features, 16) &&
       al(G.mask, 4) && al(state.h, 16) && al(state.hu, 16) && al(state.hv, 16) &&
       al(terrain_std, 16) && al(roughness_std, 16))
-    hipLaunchKernelGGL(k_features4, dim3(grid_blocks(N / 4)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(k_features4, dim3(stream_blocks(N / 4)), dim3(kBlock), 0,
                        (hipStream_t)stream, G, state.h, state.hu, state.hv, terrain_std,
                        roughness_std, nz, rain_channel, features);
   else
-    hipLaunchKernelGGL(k_features, dim3(grid_blocks(N)), dim3(kBlock), 0, (hipStream_t)stream, G,
+    hipLaunchKernelGGL(k_features, dim3(stream_blocks(N)), dim3(kBlock), 0, (hipStream_t)stream, G,
                        state.h, state.hu, state.hv, terrain_std, roughness_std, nz, rain_channel,
                        features, (long long)channel_stride, (long long)cell_stride);
   const hipError_t e = hipGetLastError();
@@ -637,12 +644,12 @@ int swe_stage_features(int32_t count, int32_t ny, int32_t nx, const float* state
   if (channel_stride == 1 && cell_stride == 6 && N % 4 == 0 && sample_stride % 4 == 0 &&
       al(features, 16) && al(state_nhwc, 16) && al(domain_mask, 4) && al(terrain_std, 16) &&
       al(roughness_std, 16)) {
-    const int nb = std::min(grid_blocks(N / 4), 1024);
+    const int nb = std::min(stream_blocks(N / 4), 65535);
     hipLaunchKernelGGL(k_stage4, dim3(nb, count), dim3(kBlock), 0, (hipStream_t)stream, N,
                        state_nhwc, domain_mask, terrain_std, roughness_std, nz, rain_channel,
                        features, (long long)sample_stride);
   } else {
-    const int nb = std::min(grid_blocks(N), 1024);
+    const int nb = std::min(stream_blocks(N), 65535);
     hipLaunchKernelGGL(k_stage, dim3(nb, count), dim3(kBlock), 0, (hipStream_t)stream, N,
                        state_nhwc, domain_mask, terrain_std, roughness_std, nz, rain_channel,
                        features, (long long)sample_stride, (long long)channel_stride,
@@ -661,6 +668,8 @@ int swe_correct_depth(int32_t count, int32_t ny, int32_t nx, const float* depth,
   if (count == 0) return SWE_OK;
   if (!depth || !residual || !out) return set_error(SWE_EINVAL, "swe_correct_depth: null argument");
   const size_t N = (size_t)nx * (size_t)ny;
+  // (a small pass, 12 B per cell: measured faster with each thread walking
+  // 16 cells than with one cell per thread, 30.7 against 34.8 us at 4096^2)
   const int nb = std::min(grid_blocks(N), 1024);
   hipLaunchKernelGGL(k_correct_depth, dim3(nb, count), dim3(kBlock), 0, (hipStream_t)stream, N,
                      depth, residual, (long long)sample_stride, (long long)cell_stride, out);

@@ -129,22 +129,22 @@ extern "C" int swe_conserved_to_primitive(SweContext* c, const float* h, const f
   if (n_times > 0) {
     if (vec_grid && aligned(h, 16) && aligned(hu, 16) && aligned(hv, 16) && aligned(depth, 16) &&
         aligned(x_velocity, 16) && aligned(y_velocity, 16) && aligned(velocity, 16)) {
-      const int nb = (int)std::max<size_t>(1, std::min<size_t>((N / 4 + kBlock - 1) / kBlock, 2048));
+      const int nb = (int)std::max<size_t>(1, std::min<size_t>((N / 4 + kBlock - 1) / kBlock, 65535));
       hipLaunchKernelGGL(k_primitive4, dim3(nb, n_times), dim3(kBlock), 0, s, G,
                          (const float4*)h, (const float4*)hu, (const float4*)hv, (float4*)depth,
                          (float4*)x_velocity, (float4*)y_velocity, (float4*)velocity);
     } else {
-      const int nb = (int)std::max<size_t>(1, std::min<size_t>((N + kBlock - 1) / kBlock, 2048));
+      const int nb = (int)std::max<size_t>(1, std::min<size_t>((N + kBlock - 1) / kBlock, 65535));
       hipLaunchKernelGGL(k_primitive, dim3(nb, n_times), dim3(kBlock), 0, s, G, h, hu, hv, depth,
                          x_velocity, y_velocity, velocity);
     }
   }
   if (elevation) {
     if (vec_grid && aligned(G.bed, 16) && aligned(elevation, 16)) {
-      const int nb = (int)std::max<size_t>(1, std::min<size_t>((N / 4 + kBlock - 1) / kBlock, 4096));
+      const int nb = (int)std::max<size_t>(1, std::min<size_t>((N / 4 + kBlock - 1) / kBlock, 65535));
       hipLaunchKernelGGL(k_elevation4, dim3(nb), dim3(kBlock), 0, s, G, (float4*)elevation);
     } else {
-      const int nb = (int)std::max<size_t>(1, std::min<size_t>((N + kBlock - 1) / kBlock, 4096));
+      const int nb = (int)std::max<size_t>(1, std::min<size_t>((N + kBlock - 1) / kBlock, 65535));
       hipLaunchKernelGGL(k_elevation, dim3(nb), dim3(kBlock), 0, s, G, elevation);
     }
   }
