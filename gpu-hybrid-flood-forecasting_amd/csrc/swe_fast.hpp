@@ -415,7 +415,8 @@ __device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, i
     r.c = __builtin_amdgcn_raw_buffer_load_b8(io.sx, io.offcls, so, 0);
     r.m = mc;
   }
-  if constexpr (LAY != kStaticCls) r.a = !signbit(r.n);  // sign bit clear: active
+  // r.a is set where the row is first used (settled): taken here, at the
+  // load, its comparison made the compiler wait for this newest load at once.
   r.neg_hv = EDGE && ((j < 0 && G.bt[2] == 0) || (j >= G.ny && G.bt[3] == 0));
   return r;
 }
@@ -425,10 +426,8 @@ __device__ __forceinline__ RawCell load_raw(const Grid& G, const PackedIO& io, i
 // behind it stay in flight.
 template <int LAY, class IO>
 __device__ __forceinline__ RawCell settled(RawCell r, const IO& io) {
-  if constexpr (LAY == kStaticCls) {
-    r.n = io.tab[r.c];
-    r.a = !signbit(r.n);
-  }
+  if constexpr (LAY == kStaticCls) r.n = io.tab[r.c];
+  if constexpr (std::is_same_v<IO, PackedIO>) r.a = !signbit(r.n);  // sign bit clear: active
   return r;
 }
 
