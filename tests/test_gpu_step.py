@@ -15,7 +15,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import load_golden, oracle_params_from_golden, random_case
+from conftest import load_golden, random_case
 
 pytestmark = pytest.mark.gpu
 
