@@ -26,9 +26,10 @@ frozen-schedule finite-difference check) against the plain step; the FP64
 mass balance of each benchmark configuration (configurations 2 and 4 also
 with the rain source rounded stochastically, ``swe_set_rain_rounding``); and,
 in the ``cpu_baseline``
-leg, the NumPy FP32 oracle timed on the host (a few steps of the 4096^2
-workload, and the 256^2 sloped-plane configuration in full, median of reps)
-together with the accuracy record of the GPU's run of that configuration
+leg, the FP32 oracle's step timed on the host: its C/OpenMP restatement on
+every host thread (the reported value; ``oracle/swe_step_omp.c``) and the
+NumPy oracle on one thread, each over a few steps of the 4096^2 workload and
+the 256^2 sloped-plane configuration in full (median of reps), together with the accuracy record of the GPU's run of that configuration
 against the oracle (per-field max-abs and RMSE against the FP64 oracle,
 wet-mask mismatches against the FP32 oracle).
 
@@ -270,16 +271,23 @@ def config1_gpu_run(variant: str) -> dict:
             "variant": variant}
 
 
-def cpu_baseline(case, steps: int, cfg1: dict, reps: int) -> tuple[dict, dict]:
-    """The NumPy FP32 oracle on the host, one thread: ``steps`` adaptive steps
-    of the 4096^2 workload (the reported value), and the 256^2 configuration
-    in full (median of ``reps``).  The oracle is also the checker of the
-    GPU's run of that configuration: replayed in float32 and float64 along the
-    GPU's own dt schedule, it gives the accuracy record."""
+def cpu_baseline(case, steps: int, cfg1: dict, reps: int, omp_steps: int) -> tuple[dict, dict]:
+    """The oracle's float32 step on the host: the reported value is the C/OpenMP
+    restatement (oracle/swe_step_omp.c, every host thread) over ``omp_steps``
+    adaptive steps of the 4096^2 workload, beside the NumPy oracle on one
+    thread over ``steps`` steps; the 256^2 configuration runs in full with both
+    (median of ``reps``).  The NumPy oracle is also the checker of the GPU's run
+    of that configuration: replayed in float32 and float64 along the GPU's own
+    dt schedule, it gives the accuracy record."""
+    import subprocess
+
     import numpy as np
 
     sys.path.insert(0, ROOT)
     from oracle import swe_oracle as o
+    from oracle import swe_step_omp as comp
+
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
 
     p = o.OracleParams(bed=case.bed, manning_n=case.manning_n, domain_mask=case.domain_mask,
                        boundary_types=case.boundary_types, rainfall_times_s=case.rain_times_s,
@@ -306,16 +314,42 @@ def cpu_baseline(case, steps: int, cfg1: dict, reps: int) -> tuple[dict, dict]:
         secs.append(time.perf_counter() - t1)
         _progress(f"cpu baseline: configuration 1 rep {len(secs)}/{reps}, {secs[-1]:.1f} s")
     med = float(np.median(secs))
+    # C/OpenMP restatement: one untimed step, then the timed ones.
+    threads = comp.threads()
+    st0 = o.OracleState(case.h0, z, z)
+    comp.run_steps(st0, p, 1)
+    t0 = time.perf_counter()
+    comp.run_steps(st0, p, omp_steps)
+    el_c = time.perf_counter() - t0
+    _progress(f"cpu baseline: {omp_steps} C/OpenMP steps at {case.bed.shape[0]}^2 on {threads} "
+              f"threads, {el_c:.1f} s")
+    secs_c = []
+    rc1 = None
+    for _ in range(reps):
+        t1 = time.perf_counter()
+        rc1 = comp.integrate_to_outputs(o.OracleState(c1.h0, z1, z1), op1, cfg1["times"],
+                                        max_steps=20000)
+        secs_c.append(time.perf_counter() - t1)
+    med_c = float(np.median(secs_c))
+    cfg1_desc = ("sloped plane with bumps and a channel, 256x256, dx 2 m, dry start, "
+                 "trapezoid rain to 50 mm/h, cfl 0.9, 600 s, 11 outputs")
     base = {
-        "value": n * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-        "sample": f"{steps} adaptive steps (CFL dt + step) of the same {case.bed.shape[0]}x"
-                  f"{case.bed.shape[1]} workload, NumPy FP32 oracle, single thread, {el:.1f} s",
+        "value": n * omp_steps / el_c, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+        "sample": f"{omp_steps} adaptive steps (CFL dt + step) of the same {case.bed.shape[0]}x"
+                  f"{case.bed.shape[1]} workload, C/OpenMP restatement of the FP32 oracle "
+                  f"(oracle/swe_step_omp.c), {threads} threads, {el_c:.1f} s",
         "host_cpu_count": os.cpu_count(),
         "config1_full": {
-            "workload": "sloped plane with bumps and a channel, 256x256, dx 2 m, dry start, "
-                        "trapezoid rain to 50 mm/h, cfl 0.9, 600 s, 11 outputs",
-            "steps": int(r1.steps_used), "reps": reps, "seconds": secs, "median_s": med,
-            "cell_updates_per_s": c1.bed.size * r1.steps_used / med,
+            "workload": cfg1_desc, "steps": int(rc1[1]), "reps": reps, "seconds": secs_c,
+            "median_s": med_c, "cell_updates_per_s": c1.bed.size * rc1[1] / med_c,
+        },
+        "numpy_single_thread": {
+            "value": n * steps / el, "unit": "cell-updates/s", "cores": 1,
+            "sample": f"{steps} adaptive steps of the same workload, NumPy FP32 oracle, {el:.1f} s",
+            "config1_full": {
+                "workload": cfg1_desc, "steps": int(r1.steps_used), "reps": reps, "seconds": secs,
+                "median_s": med, "cell_updates_per_s": c1.bed.size * r1.steps_used / med,
+            },
         },
     }
     rec = accuracy_record(cfg1)
@@ -371,6 +405,38 @@ def config2_mass_balance(variant: str, rain_rounding: str = "nearest") -> dict:
     return {"rain_rounding": rain_rounding, "outputs_written": int(r.outputs_written),
             "steps": int(r.steps_used), "wall_s": wall,
             "mass_balance_rel": d["mass_balance_rel"], "nonfinite": d["nonfinite"]}
+
+
+def cpu_config2_full() -> dict:
+    """The 1024^2 urban configuration in full (10800 s, 181 outputs) with the
+    C/OpenMP restatement of the FP32 oracle on every host thread (about a
+    minute on 16 EPYC threads; opt-in with --cpu-config2)."""
+    import subprocess
+
+    import numpy as np
+
+    sys.path.insert(0, ROOT)
+    from hybrid_flood_amd import synthetic as sy
+    from oracle import swe_oracle as o
+    from oracle import swe_step_omp as comp
+
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    u = sy.urban_case(1024)
+    p = o.OracleParams(bed=u.bed, manning_n=u.manning_n, domain_mask=u.domain_mask,
+                       boundary_types=u.boundary_types, rainfall_times_s=u.rain_times_s,
+                       rainfall_rates_m_s=u.rain_rates_m_s, dx=u.dx, cfl=u.cfl,
+                       dry_tolerance=u.dry_tolerance, max_dt=u.max_dt)
+    z = np.zeros_like(u.h0)
+    times = np.arange(0.0, 10801.0, 60.0).astype(np.float32)
+    t0 = time.perf_counter()
+    _, steps, t, written = comp.integrate_to_outputs(o.OracleState(u.h0, z, z), p, times,
+                                                     max_steps=200000)
+    el = time.perf_counter() - t0
+    _progress(f"cpu baseline: configuration 2 in full, {steps} steps, {el:.1f} s")
+    return {"workload": "urban 1024x1024, dx 2 m, one open edge, 1-min rain over 10800 s, "
+                        "181 outputs", "threads": comp.threads(), "steps": steps,
+            "outputs_written": written, "final_time_s": t, "seconds": el,
+            "cell_updates_per_s": u.bed.size * steps / el}
 
 
 def bench_ensemble(variant: str, count: int, seed: int, t_end: float, plain_cups: float):
@@ -670,6 +736,10 @@ def main(argv=None) -> int:
     ap.add_argument("--variant", default="fast", choices=["fast", "reference"])
     ap.add_argument("--rows", type=int, default=0, help="rows per wavefront strip (0 = autotune)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-omp-steps", type=int, default=50,
+                    help="steps of the C/OpenMP restatement for cpu_baseline's value")
+    ap.add_argument("--cpu-config2", action="store_true",
+                    help="also run the 1024^2 configuration in full with the C/OpenMP oracle")
     ap.add_argument("--cpu-reps", type=int, default=5,
                     help="timed repetitions of the oracle on the 256^2 configuration")
     ap.add_argument("--extra", action="store_true", help="also time the other kernel variant")
@@ -813,7 +883,9 @@ def main(argv=None) -> int:
             cfg1 = config1_gpu_run(args.variant)
         _progress("cpu baseline: timing the oracle on the host")
         line["cpu_baseline"], line["accuracy"] = cpu_baseline(case, args.cpu_steps, cfg1,
-                                                              args.cpu_reps)
+                                                              args.cpu_reps, args.cpu_omp_steps)
+        if args.cpu_config2:
+            line["cpu_baseline"]["config2_full"] = cpu_config2_full()
         other = "reference" if args.variant == "fast" else "fast"
         line[f"accuracy_{other}_variant"] = accuracy_record(config1_gpu_run(other))
         _progress(f"accuracy record of the {other} variant done")
