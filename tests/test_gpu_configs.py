@@ -9,7 +9,10 @@ and a 512^2 crop of the 4096^2 roofline workload against the FP64 oracle.
   MI355X: 1.3e-7 / 4e-8 for the reference variant, 3.2e-7 / 3.5e-6 for the
   fast one); FP64 mass balance below 1e-6.
 * Urban DEM, 1024^2, dx 2 m, one open (east) edge, 1-min rain series over
-  10800 s, 181 outputs: too long for the oracle, so size-independent
+  10800 s, 181 outputs: its first 3000 s (1339 steps) against the C/OpenMP
+  transcription of the FP32 oracle (same step count and final time, no
+  wet-mask mismatch, depth within 2e-6 m, momentum within 5e-7 m^2/s); the
+  whole run, too long for a test's oracle, through size-independent
   properties -- every output written, final time, no non-finite value,
   depth >= 0, outflow only through the open edge, the rain volume within 1e-3
   of the hyetograph's integral (the loop applies the rate at each step's
@@ -268,3 +271,59 @@ def test_one_loop_step_at_4096(cuda, dam_break_4096_after_30, variant, rows, wav
         # RMSE 1.78e-8 on hu for both)
         assert float(err.max()) <= 1.05 * float(err32.max()) + 1e-9, f
         assert rmse <= 1.1 * rmse32 + 1e-12, f
+
+
+# Max |GPU - C oracle| after the 1339 steps of the first 3000 s, measured on
+# MI355X: reference variant h 1.6e-7 m, hu 2.3e-8, hv 1.9e-8 m^2/s; fast
+# variant 2.6e-7, 6.4e-8, 3.4e-8; no wet-mask mismatch.  The bars leave ~8x.
+URBAN_WINDOW_TOL = {"h": 2e-6, "hu": 5e-7, "hv": 5e-7}
+
+
+@pytest.fixture(scope="module")
+def urban_first_window_oracle():
+    """The 1024^2 urban configuration's first 3000 s (50 outputs, ~1300 steps:
+    the rain wets two thirds of the cells and the water starts to run), by the
+    C/OpenMP transcription of the FP32 oracle on the host's threads (seconds;
+    the NumPy oracle would take minutes)."""
+    import subprocess
+    from pathlib import Path
+
+    from hybrid_flood_amd import synthetic as sy
+    from oracle import swe_oracle as o
+
+    subprocess.run(["make", "-s", "-C", str(Path(__file__).resolve().parents[1] / "oracle")],
+                   check=True)
+    from oracle import swe_step_omp as comp
+
+    u = sy.urban_case(1024)
+    op = o.OracleParams(bed=u.bed, manning_n=u.manning_n, domain_mask=u.domain_mask,
+                        boundary_types=u.boundary_types, rainfall_times_s=u.rain_times_s,
+                        rainfall_rates_m_s=u.rain_rates_m_s, dx=u.dx, cfl=u.cfl,
+                        dry_tolerance=u.dry_tolerance, max_dt=u.max_dt)
+    times = np.arange(0.0, 3001.0, 60.0).astype(np.float32)
+    z = np.zeros_like(u.h0)
+    print(f"urban 1024 first 3000 s: C oracle on {comp.threads()} threads", flush=True)
+    last, steps, t, written = comp.integrate_to_outputs(o.OracleState(u.h0, z, z), op, times,
+                                                        max_steps=200000)
+    return u, times, last, steps, t, written
+
+
+@pytest.mark.parametrize("variant", ["reference", "fast"])
+def test_urban_1024_first_window_against_c_oracle(cuda, urban_first_window_oracle, variant):
+    from hybrid_flood_amd import numerics as nm
+
+    u, times, last, steps, t, written = urban_first_window_oracle
+    r = nm.integrate_to_outputs(_zero_state(u, cuda), _params(u, cuda), times, max_steps=200000,
+                                variant=variant)
+    got = {f: getattr(r.states, f)[-1].cpu().numpy() for f in ("h", "hu", "hv")}
+    want = {"h": last.h, "hu": last.hu, "hv": last.hv}
+    diffs = {f: float(np.abs(got[f] - want[f]).max()) for f in got}
+    wet = (got["h"] > u.dry_tolerance) != (want["h"] > u.dry_tolerance)
+    print(f"{variant}: steps {int(r.steps_used)} vs {steps}, max diffs {diffs}, "
+          f"wet-mask mismatches {int(wet.sum())}", flush=True)
+    assert int(r.outputs_written) == written == 51
+    assert int(r.steps_used) == steps
+    assert float(r.final_time_s) == t
+    assert int(wet.sum()) == 0
+    for f, bar in URBAN_WINDOW_TOL.items():
+        assert diffs[f] <= bar, f
