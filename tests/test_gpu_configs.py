@@ -21,6 +21,10 @@ and a 512^2 crop of the 4096^2 roofline workload against the FP64 oracle.
   (tests/test_oracle_mass_balance.py; measured 2.5e-6 against ~1.3e-5).
 * Dam-break urban crop, 512^2 of the 4096^2 roofline workload, 7 s: both
   variants against the FP64 oracle stepped with the GPU's dt schedule.
+* The 4096^2 roofline workload itself: one loop step at the benchmark's tile
+  against the FP32 and FP64 oracles, and its first 100 adaptive steps (each
+  side taking its own CFL dt) against the C/OpenMP transcription of the FP32
+  oracle.
 """
 
 from __future__ import annotations
@@ -326,4 +330,67 @@ def test_urban_1024_first_window_against_c_oracle(cuda, urban_first_window_oracl
     assert float(r.final_time_s) == t
     assert int(wet.sum()) == 0
     for f, bar in URBAN_WINDOW_TOL.items():
+        assert diffs[f] <= bar, f
+
+
+# After 100 steps (7.1 s), measured on MI355X: simulated time within 6.2e-9
+# (fast) and 4.8e-10 (reference) relative of the oracle's; max |GPU - oracle|
+# fast h 1.06e-6 m, hu 1.64e-6, hv 1.19e-6 m^2/s; reference 7.7e-7, 1.25e-6,
+# 1.31e-6; no wet-mask mismatch.  The bars leave ~10x.
+DAM_BREAK_100_TOL = {"h": 1e-5, "hu": 1.5e-5, "hv": 1.5e-5}
+
+
+@pytest.fixture(scope="module")
+def dam_break_4096_c_oracle_100():
+    """The 4096^2 roofline workload's first 100 adaptive steps (CFL dt, then
+    the step) by the C/OpenMP transcription of the FP32 oracle on the host's
+    threads."""
+    import subprocess
+    from pathlib import Path
+
+    from hybrid_flood_amd import synthetic as sy
+    from oracle import swe_oracle as o
+
+    subprocess.run(["make", "-s", "-C", str(Path(__file__).resolve().parents[1] / "oracle")],
+                   check=True)
+    from oracle import swe_step_omp as comp
+
+    c = sy.dam_break_urban_case(4096)
+    op = o.OracleParams(bed=c.bed, manning_n=c.manning_n, domain_mask=c.domain_mask,
+                        boundary_types=c.boundary_types, rainfall_times_s=c.rain_times_s,
+                        rainfall_rates_m_s=c.rain_rates_m_s, dx=c.dx, cfl=c.cfl,
+                        dry_tolerance=c.dry_tolerance, max_dt=c.max_dt)
+    z = np.zeros_like(c.h0)
+    print(f"4096^2: 100 C oracle steps on {comp.threads()} threads", flush=True)
+    last, t = comp.run_steps(o.OracleState(c.h0, z, z), op, 100)
+    return c, last, t
+
+
+@pytest.mark.parametrize("variant,rows", [("fast", 15), ("reference", 0)])
+def test_loop_100_steps_at_4096_against_c_oracle(cuda, dam_break_4096_c_oracle_100, variant,
+                                                 rows):
+    """100 adaptive steps of the device loop on the full 4096^2 grid (the
+    packed fast loop at the benchmark's 15-row tile, and the reference-order
+    loop) against the oracle's 100 steps, each side taking its own CFL dt."""
+    from hybrid_flood_amd import numerics as nm
+
+    c, last, t = dam_break_4096_c_oracle_100
+    params = _params(c, cuda)
+    ctx = nm.context_for(params, variant)
+    ctx.set_tile_rows(rows)
+    try:
+        st = nm.SWEState(*(x.clone() for x in _zero_state(c, cuda)))
+        info = nm.run_steps(st, params, 100, variant=variant, diagnostics=True)
+    finally:
+        ctx.set_tile_rows(0)
+    got = {f: getattr(st, f).cpu().numpy() for f in ("h", "hu", "hv")}
+    diffs = {f: float(np.abs(got[f] - getattr(last, f)).max()) for f in got}
+    wet = (got["h"] > np.float32(c.dry_tolerance)) != (last.h > np.float32(c.dry_tolerance))
+    dt_rel = abs(float(info["final_time_s"]) - t) / t
+    print(f"{variant}: time {info['final_time_s']!r} vs {t!r} (rel {dt_rel:.2e}), max diffs "
+          f"{diffs}, wet-mask mismatches {int(wet.sum())}", flush=True)
+    assert info["steps_used"] == 100 and info["nonfinite"] == 0
+    assert dt_rel <= 1e-7
+    assert int(wet.sum()) == 0
+    for f, bar in DAM_BREAK_100_TOL.items():
         assert diffs[f] <= bar, f
