@@ -1,5 +1,6 @@
 """The C/OpenMP restatement of the step (``oracle/swe_step_omp.c``) against
-the NumPy oracle: a second, independent transcription of the same arithmetic.
+the NumPy oracle, a second, independent transcription of the same
+arithmetic, and against the V1 golden vectors.
 
 Depth and the CFL step size agree bit for bit on every case.  Momentum agrees
 bit for bit where the friction term's power is the same function: with no
@@ -140,3 +141,25 @@ def test_output_loop_matches_oracle_without_friction(c):
     assert (steps, t, written) == (ref.steps_used, ref.final_time_s, ref.outputs_written)
     for x, y in zip(last, ref.states):
         assert np.array_equal(x, y[-1])
+
+
+GOLDEN_STEPS = ["v1_step_parity_7x9.npz", "v1_step_wetdry_16x12.npz", "v1_step_walls_20x24.npz"]
+
+
+@pytest.mark.parametrize("name", GOLDEN_STEPS)
+def test_c_step_against_v1_golden_vectors(c, name):
+    """Straight against the V1 golden vectors (written from the reference's
+    NumPy twin): the CFL dt and the depth bit for bit; momentum, where V1 takes
+    np.hypot and V2's (and this) sqrt(hu^2 + hv^2), within the reference's own
+    V1/V2 bar (tests/test_v1_v2_architecture.py:104)."""
+    from conftest import load_golden, oracle_params_from_golden
+    from oracle import swe_oracle as o
+
+    z = load_golden(name)
+    p = oracle_params_from_golden(z)
+    st = o.OracleState(z["h"], z["hu"], z["hv"])
+    assert c.compute_cfl_dt(st, p) == float(z["cfl_dt"])
+    out = c.finite_volume_step(st, p, float(z["dt"]), float(z["t"]))
+    assert np.array_equal(out.h, z["out_h"])
+    np.testing.assert_allclose(out.hu, z["out_hu"], rtol=2e-5, atol=2e-7)
+    np.testing.assert_allclose(out.hv, z["out_hv"], rtol=2e-5, atol=2e-7)
